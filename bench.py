@@ -1,0 +1,135 @@
+"""Throughput of the WaveQ3D hot path on one MI355X.
+
+Workload (default): the range-dependent single-fan case - gridded 3-D sound
+speed (Mackenzie on seeded T/S, pchip x linear x linear) and 1/60 deg
+bathymetry, DE -80:0.25:80 x AZ 0:2:358 = 115,380 rays, 8 frequencies,
+100 x 100 targets, dt 0.1 s.  A "step" is one wave_queue::step() of the whole
+fan: reflections, AB3 propagation, eigenray search over all targets and
+eigenray construction.  W warm-up steps run untimed, then exactly K steps are
+timed; the default W + K = 1201 covers the whole 120 s run.
+
+Prints one JSON line: ray-steps/s of the timed steps (eigenrays/s beside it),
+the roofline of the dominant kernel measured with HIP events on the library's
+stream, and the CPU oracle timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+W_TGT = 17.0              # FP64 ops per (target, ray, step) of the distance test
+KERNELS = ["k_init", "k_reflect", "k_advance", "k_detect", "k_build"]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(steps=2):
+    """Oracle (scalar C++ port, 1 thread) on `steps` steps of the same fan,
+    environment and targets."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from underseamodelinglibrary_amd import scenarios
+    sc = scenarios.config2()
+    q = oracle.oracle_queue(*sc.args())
+    t0 = time.perf_counter()
+    q.step(steps)
+    dt = time.perf_counter() - t0
+    n = sc.num_rays()
+    return {"value": n * steps / dt, "unit": "ray-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle, steps 1-{steps} of the full fan with all {sc.num_targets()} "
+                      f"targets ({dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1181)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(independent fans or target subsets would shard with no exchange)", file=sys.stderr)
+        sys.exit(2)
+
+    from underseamodelinglibrary_amd import _lib, scenarios
+    from underseamodelinglibrary_amd.usml.waveq3d import wave_queue
+    import ctypes as C
+
+    lib = _lib.load()
+    if lib.wq3d_device_count() < 1:
+        raise RuntimeError("no HIP device")
+    log("[bench] building config-2 environment")
+    sc = scenarios.config2()
+    log(f"[bench] fan {sc.num_rays()} rays, {sc.num_targets()} targets, "
+        f"{sc.freq.size()} frequencies; warmup {args.warmup}, timed {args.steps}")
+    q = wave_queue(*sc.args())
+    q.step(args.warmup)
+    q.sync()
+    q.drain_eigenrays()
+    lib.check(lib.wq3d_set_timing(q._fan, 1))
+    lib.check(lib.wq3d_kernel_times(q._fan, (C.c_double * 8)(), (C.c_int64 * 8)(), 8))
+    q.sync()
+    t0 = time.perf_counter()
+    done = 0
+    chunk = 100
+    while done < args.steps:
+        n = min(chunk, args.steps - done)
+        q.step(n)
+        done += n
+        log(f"[bench] {args.warmup + done} steps")
+    q.sync()
+    elapsed = time.perf_counter() - t0
+    ms = (C.c_double * 8)()
+    launches = (C.c_int64 * 8)()
+    lib.check(lib.wq3d_kernel_times(q._fan, ms, launches, 8))
+    rays, _, _ = q.drain_eigenrays()
+    n_rays = sc.num_rays()
+    value = n_rays * args.steps / elapsed
+    kernel_ms = {KERNELS[k]: ms[k] for k in range(5) if launches[k]}
+    # dominant kernel: the target search
+    det_ms = ms[3]
+    flops = W_TGT * sc.num_targets() * n_rays
+    achieved = flops / (det_ms * 1e-3) / 1e12 if det_ms > 0 else 0.0
+    out = {
+        "metric": "ray-steps/sec (plain per-fan kernel)",
+        "value": value,
+        "unit": "ray-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded T/S fields and seamount bathymetry)",
+        "config": {"workload": "range-dependent single fan: 641 DE x 180 AZ rays, 8 freqs, "
+                               "100x100 targets, 3-D SSP grid + 1/60 deg bathymetry, dt 0.1 s",
+                   "rays": n_rays, "targets": sc.num_targets(), "frequencies": sc.freq.size()},
+        "eigenrays": int(len(rays)),
+        "eigenrays_per_s": len(rays) / elapsed,
+        "kernel_ms_per_launch": kernel_ms,
+        "roofline": {"bound": "mfma", "kernel": "k_detect", "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "note": "FP64 vector; 17 FP64 ops per (target, ray) pair per launch"},
+    }
+    q.close()
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle)")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,123 @@
+"""HIP library vs CPU oracle on the same seeded inputs, through the C ABI.
+
+Bar: identical eigenray keys (step, t1, t2, de, az) and integer counters,
+travel time within 1e-9 relative, intensity within 0.01 dB, angles within
+1e-6 deg; wavefront state within 1e-10 scaled; eigenverb / reflection
+records with identical keys and values within 1e-9.
+"""
+import numpy as np
+import pytest
+
+from underseamodelinglibrary_amd import _abi, scenarios
+from underseamodelinglibrary_amd.usml import ocean as oc
+from underseamodelinglibrary_amd.usml.types import seq_linear, seq_log
+from underseamodelinglibrary_amd.usml.waveq3d import wave_queue
+
+from parity import compare_eigenrays, compare_fronts
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(sc, oracle_mod, steps=None, verbs=False, refl=False):
+    steps = sc.steps if steps is None else steps
+    print(f"[parity] {sc.name}: {sc.num_rays()} rays, {sc.num_targets()} targets, {steps} steps",
+          flush=True)
+    g = wave_queue(*sc.args())
+    o = oracle_mod.oracle_queue(*sc.args())
+    dummy = type("L", (), {"add_eigenverb": lambda *a: None, "reflect": lambda *a: None})()
+    if verbs:
+        g.add_eigenverb_listener(dummy)
+        o.add_eigenverb_listener(dummy)
+    if refl:
+        g.add_reflection_listener(dummy)
+        o.add_reflection_listener(dummy)
+    # listeners are replayed by step(); records are compared raw, so drive the
+    # raw stepping entry points without per-step replay
+    g._lib.check(g._lib.wq3d_step(g._fan, steps))
+    o._lib.orc_step(o._h, steps)
+    return g, o
+
+
+def _check_all(g, o, nf):
+    gr, gi, gp = g.drain_eigenrays()
+    orr, oi, op = o.drain_eigenrays()
+    n = compare_eigenrays(gr, gi, gp, orr, oi, op, nf)
+    for slot in (_abi.PAST, _abi.PREV, _abi.CURR, _abi.NEXT):
+        compare_fronts(g.snapshot_arrays(slot), o.snapshot_arrays(slot))
+    return n
+
+
+def test_eigenray_basic_parity(oracle_mod):
+    sc = scenarios.eigenray_basic()
+    g, o = _run(sc, oracle_mod)
+    assert _check_all(g, o, sc.freq.size()) == 3
+
+
+def test_config1_full_parity(oracle_mod):
+    """Config 1 at its full size: 121 x 36 rays, 100 targets, 600 steps."""
+    sc = scenarios.config1()
+    g, o = _run(sc, oracle_mod)
+    n = _check_all(g, o, sc.freq.size())
+    assert n > 1000
+
+
+def test_config2_reduced_parity(oracle_mod):
+    """Gridded sound speed (pchip x linear x linear) and bathymetry with a
+    reduced fan and target grid."""
+    sc = scenarios.config2(de=seq_linear(-80.0, 2.0, 80.0), az=seq_linear(0.0, 10.0, 350.0),
+                           n_targets=(10, 10), time_max=30.0, freq=seq_log(100.0, 10.0, 3))
+    g, o = _run(sc, oracle_mod)
+    assert _check_all(g, o, sc.freq.size()) > 100
+
+
+def test_eigenverb_and_reflection_parity(oracle_mod):
+    """Eigenverbs at every surface/bottom interaction plus a volume layer."""
+    sc = scenarios.config1(de=seq_linear(-60.0, 3.0, 60.0), az=seq_linear(0.0, 30.0, 330.0),
+                           time_max=20.0)
+    sc.ocean.add_volume(oc.volume_flat(300.0, 10.0))
+    g, o = _run(sc, oracle_mod, verbs=True, refl=True)
+    _check_all(g, o, sc.freq.size())
+    gv, gpw = g.drain_eigenverbs()
+    ov, opw = o.drain_eigenverbs()
+    assert len(gv) == len(ov) and len(ov) > 100
+    for k in ("step", "de", "az", "seq", "type", "surface", "bottom", "caustic", "upper", "lower"):
+        np.testing.assert_array_equal(gv[k], ov[k], err_msg=k)
+    for k in ("travel_time", "length", "width", "direction", "grazing", "sound_speed",
+              "pos_rho", "pos_theta", "pos_phi", "source_de", "source_az"):
+        scale = np.maximum(np.abs(ov[k]), 1.0)
+        assert (np.abs(gv[k] - ov[k]) / scale).max() <= 1e-9, k
+    assert (np.abs(gpw - opw) / np.maximum(np.abs(opw), 1e-300)).max() <= 1e-9
+    gf = g.drain_reflections()
+    of = o.drain_reflections()
+    assert len(gf) == len(of) and len(of) > 100
+    for k in ("step", "de", "az", "seq", "type"):
+        np.testing.assert_array_equal(gf[k], of[k], err_msg=k)
+    for k in ("time", "dt", "grazing", "speed", "pos_rho", "ndir_rho", "ndir_theta"):
+        scale = np.maximum(np.abs(of[k]), 1.0)
+        assert (np.abs(gf[k] - of[k]) / scale).max() <= 1e-9, k
+
+
+def test_listener_replay_order(oracle_mod):
+    """Listener calls arrive in the reference's order with identical content."""
+    sc = scenarios.eigenray_basic()
+
+    class L:
+        def __init__(self):
+            self.calls = []
+
+        def add_eigenray(self, t1, t2, ray, run_id):
+            self.calls.append(("ray", t1, t2, round(ray.travel_time, 9)))
+
+        def check_eigenrays(self, time, run_id):
+            self.calls.append(("check", round(time, 9)))
+
+    lg, lo = L(), L()
+    g = wave_queue(*sc.args())
+    o = oracle_mod.oracle_queue(*sc.args())
+    g.add_eigenray_listener(lg)
+    o.add_eigenray_listener(lo)
+    while g.time() < 3.5:
+        g.step()
+        o.step()
+    assert lg.calls == lo.calls
+    assert sum(1 for c in lg.calls if c[0] == "ray") == 3

@@ -1,0 +1,160 @@
+"""Pins the CPU oracle against the reference's own known-answer tests.
+
+Each test restates the scenario and the assertions of a reference test
+(file:line cited) and runs it on the oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from underseamodelinglibrary_amd import scenarios
+from underseamodelinglibrary_amd.usml import ocean as oc
+from underseamodelinglibrary_amd.usml.types import (compute_earth_radius, seq_linear, seq_log,
+                                                    set_earth_radius, wposition, wposition1)
+
+
+class collector:
+    def __init__(self):
+        self.rays = []
+
+    def add_eigenray(self, t1, t2, ray, run_id):
+        self.rays.append((t1, t2, ray))
+
+    def check_eigenrays(self, time, run_id):
+        pass
+
+
+def test_eigenray_basic(oracle_mod):
+    """waveq3d/test/eigenray_test.cc:86-198: direct, surface and bottom paths."""
+    sc = scenarios.eigenray_basic()
+    q = oracle_mod.oracle_queue(*sc.args())
+    c = collector()
+    q.add_eigenray_listener(c)
+    while q.time() < 3.5:
+        q.step()
+    assert len(c.rays) == 3
+    direct, surf, bot = (r for _, _, r in c.rays)
+    assert abs(direct.intensity[0] - 66.9506) < 0.1
+    assert abs(direct.travel_time - 1.484018789) < 0.002
+    assert abs(direct.phase[0] - 0.0) < 1e-6
+    assert abs(direct.source_de + 0.01) < 0.01
+    assert abs(direct.target_de - 0.01) < 0.01
+    assert abs(surf.intensity[0] - 69.5211) < 0.1
+    assert abs(surf.travel_time - 1.995102731) < 0.002
+    assert abs(surf.phase[0] + math.pi) < 1e-6
+    assert abs(surf.source_de - 41.93623171) < 0.01
+    assert abs(surf.target_de + 41.93623171) < 0.01
+    assert abs(bot.intensity[0] - 73.2126) < 4.0
+    assert abs(bot.travel_time - 3.051676949) < 0.02
+    assert abs(bot.phase[0] - 0.0) < 1e-6
+    for r in (direct, surf, bot):
+        assert abs(r.source_az) < 1e-6
+        assert abs(r.target_az) < 1e-6
+    assert (direct.surface, direct.bottom) == (0, 0)
+    assert (surf.surface, surf.bottom) == (1, 0)
+    assert (bot.surface, bot.bottom) == (0, 1)
+
+
+class reflection_callback:
+    def __init__(self):
+        self.count = 0
+        self.events = []
+
+    def reflect(self, time, de, az, dt, grazing, speed, position, ndirection, type):
+        self.count += 1
+        self.events.append((time, position.latitude(), type))
+
+
+def test_reflect_flat(oracle_mod):
+    """waveq3d/test/reflection_test.cc:168-280: a single ray bouncing between
+    a flat surface and a 1000 m flat bottom; reflection k happens at
+    k * 7.450560973 s and latitude 45 + 0.1 k."""
+    compute_earth_radius(45.0)
+    profile = oc.profile_linear()
+    surface = oc.boundary_flat()
+    bottom = oc.boundary_flat(1000.0)
+    ocean = oc.ocean_model(surface, bottom, profile)
+    freq = seq_log(10.0, 10.0, 1)
+    pos = wposition1(45.0, -45.0, 0.0)
+    de = seq_linear(-5.183617057, 0.0, 1)
+    az = seq_linear(0.0, 0.0, 1)
+    q = oracle_mod.oracle_queue(ocean, freq, pos, de, az, 0.1)
+    cb = reflection_callback()
+    q.add_reflection_listener(cb)
+    while q.time() < 60.0:
+        q.step()
+    assert cb.count >= 7
+    for k, (t, lat, typ) in enumerate(cb.events, start=1):
+        assert abs(t - k * 7.450560973) < 1e-4
+        assert abs(lat - (45.0 + k * 0.1)) < 1e-6
+    # the source was clamped 0.1 m below the surface (wave_queue.cc:58-68)
+    assert abs(q.source_pos().altitude() + 0.1) < 1e-9
+
+
+def test_refraction_munk_range(oracle_mod):
+    """waveq3d/test/refraction_test.cc:827-955: cycle ranges of the Munk
+    profile on a flat earth within 0.01 %, one upper and one lower vertex
+    per cycle."""
+    cycle_ranges = [
+        64977.771509, 62686.699943, 60536.790347, 58539.834823, 56706.277890,
+        55044.418981, 53559.948084, 52255.876772, 51132.827760, 50189.572079,
+        49423.683193, 48832.195747, 48412.185973, 48161.238557, 48077.771909,
+        48161.238557, 48412.185973, 48832.195747, 49423.683193, 50189.572079,
+        51132.827760, 52255.876772, 53559.948084, 55044.418981, 56706.277890,
+        58539.834823, 60536.790347, 62686.699943, 64977.771509]
+    set_earth_radius(6371000.0)
+    ssp = oc.profile_munk()
+    ssp.flat_earth(True)
+    ocean = oc.ocean_model(oc.boundary_flat(), oc.boundary_flat(5000.0), ssp)
+    lat1, lng1, alt1 = 45.0, -45.0, -1000.0
+    pos = wposition1(lat1, lng1, alt1)
+    de = seq_linear(-14.0, 1.0, 14.0)
+    az = seq_linear(0.0, 0.0, 1)
+    freq = seq_log(10e3, 1.0, 1)
+    q = oracle_mod.oracle_queue(ocean, freq, pos, de, az, 0.1)
+    nd = de.size()
+    loop = np.zeros(nd)
+    up = np.zeros(nd, dtype=int)
+    low = np.zeros(nd, dtype=int)
+    checked = 0
+    R = 6371000.0
+    while q.time() < 95.0:
+        q.step()
+        pv, cu, nx = q.prev(), q.curr(), q.next()
+        for d in range(nd):
+            Hp = pv.position.rho(d, 0) - R - alt1
+            Hc = cu.position.rho(d, 0) - R - alt1
+            Hn = nx.position.rho(d, 0) - R - alt1
+            if Hc * Hn < 0.0 and cu.ndirection.rho(d, 0) * de(d) > 0.0:
+                loop[d] += 1.0
+                Rtheory = loop[d] * cycle_ranges[d]
+                rr = lambda w: w.position.rho(d, 0) * math.radians(w.position.latitude(d, 0) - lat1)
+                Rp, Rc, Rn = rr(pv), rr(cu), rr(nx)
+                h = Hn - Hp
+                slope = (Rn - Rp) / h
+                h *= 0.5
+                curve = (Rn - 2.0 * Rc + Rp) / (h * h)
+                dx = -Hc
+                Rmodel = Rc + slope * dx + 0.5 * curve * dx * dx
+                assert abs(Rmodel - Rtheory) / Rtheory * 100.0 < 0.01
+                assert cu.upper(d, 0) - up[d] == 1
+                assert cu.lower(d, 0) - low[d] == 1
+                up[d] = cu.upper(d, 0)
+                low[d] = cu.lower(d, 0)
+                checked += 1
+    assert checked > 29
+
+
+def test_source_clamp(oracle_mod):
+    """waveq3d/test/proploss_test.cc:250-280: sources above the surface or
+    below the bottom are clamped 0.1 m inside the water column."""
+    set_earth_radius(6371000.0)
+    ocean = oc.ocean_model(oc.boundary_flat(), oc.boundary_flat(1000.0), oc.profile_linear())
+    freq = seq_linear(1000.0, 0.0, 1)
+    de = seq_linear(-10.0, 10.0, 10.0)
+    az = seq_linear(0.0, 10.0, 20.0)
+    q = oracle_mod.oracle_queue(ocean, freq, wposition1(45.0, -45.0, 10.0), de, az, 0.1)
+    assert abs(q.source_pos().altitude() + 0.1) < 1e-9
+    q = oracle_mod.oracle_queue(ocean, freq, wposition1(45.0, -45.0, -2000.0), de, az, 0.1)
+    assert abs(q.source_pos().altitude() + 1000.0 - 0.1) < 1e-6

@@ -1,0 +1,186 @@
+"""Seeded synthetic environments and fans of the benchmark workloads.
+
+Each builder returns a `scenario` with the wave_queue constructor arguments
+and the step count the reference's caller loop `while (time < t_max) step()`
+produces with `time += time_step` accumulation (wave_queue.cc:167).
+
+  config1  Munk profile (c0 1500 m/s, axis 1300 m, B 1300 m, eps 0.00737),
+           flat 5000 m Rayleigh-sand bottom, flat surface, Thorp; source
+           45N 45W 1000 m; DE -60:1:60, AZ 0:10:350, 1000 Hz; dt 0.1 s,
+           t < 60 s; 10 x 10 targets within 0.5 deg, 100-2000 m deep.
+  config2  range-dependent: 3-D sound speed grid (1/4 deg over a 10 x 10 deg
+           box, 40 depths to 5000 m) from Mackenzie on seeded T/S fields;
+           1/60 deg bathymetry from 50 seeded Gaussian seamounts/troughs on
+           a 4000 m base clipped to 1500-5500 m (bilinear gen_grid);
+           DE -80:0.25:80 (641), AZ 0:2:358 (180), 8 log-spaced frequencies
+           100-10000 Hz, dt 0.1 s, t < 120 s, 100 x 100 targets.
+The T/S construction: T = 25 exp(-z/800) + 2 + dT, S = 35 + dS, where dT and
+dS are sums of 64 Gaussian bumps (random centres in the box, widths 0.5-2 deg
+horizontally and 200-1500 m vertically) rescaled to RMS 0.5 degC and 0.2 ppt.
+"""
+import math
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from .usml import ocean as oc
+from .usml.types import (compute_earth_radius, earth_radius, seq_linear, seq_log, set_earth_radius,
+                         to_colatitude, to_radians, wposition, wposition1)
+
+
+@dataclass
+class scenario:
+    name: str
+    ocean: object
+    freq: object
+    pos: object
+    de: object
+    az: object
+    time_step: float
+    targets: Optional[object]
+    steps: int
+    notes: dict = field(default_factory=dict)
+
+    def num_rays(self):
+        return self.de.size() * self.az.size()
+
+    def num_targets(self):
+        return 0 if self.targets is None else self.targets.size1() * self.targets.size2()
+
+    def args(self):
+        return (self.ocean, self.freq, self.pos, self.de, self.az, self.time_step, self.targets)
+
+
+def loop_steps(time_step, time_max):
+    """Number of step() calls made by `while (time < time_max)`."""
+    t, n = 0.0, 0
+    while t < time_max:
+        t += time_step
+        n += 1
+    return n
+
+
+def target_grid(lat0, lng0, half_width_deg, n1, n2, depth_min, depth_max):
+    """n1 x n2 targets on a uniform lat/lon grid within +-half_width of
+    (lat0, lng0); depth varies linearly with the row index."""
+    lats = np.linspace(lat0 - half_width_deg, lat0 + half_width_deg, n2)
+    lngs = np.linspace(lng0 - half_width_deg, lng0 + half_width_deg, n1)
+    depths = np.linspace(depth_min, depth_max, n1)
+    lat = np.empty((n1, n2))
+    lng = np.empty((n1, n2))
+    alt = np.empty((n1, n2))
+    for a in range(n1):
+        for b in range(n2):
+            lat[a, b] = lats[b]
+            lng[a, b] = lngs[a]
+            alt[a, b] = -depths[a]
+    return wposition.from_arrays(lat, lng, alt)
+
+
+def config1(de=None, az=None, n_targets=(10, 10), time_max=60.0, targets=True):
+    set_earth_radius(6371000.0)
+    profile = oc.profile_munk(1300.0, 1300.0, 1500.0, 0.00737, oc.attenuation_thorp())
+    bottom = oc.boundary_flat(5000.0, oc.reflect_loss_rayleigh("sand"))
+    surface = oc.boundary_flat()
+    ocean = oc.ocean_model(surface, bottom, profile)
+    freq = seq_linear(1000.0, 0.0, 1)
+    pos = wposition1(45.0, -45.0, -1000.0)
+    de = de if de is not None else seq_linear(-60.0, 1.0, 60.0)
+    az = az if az is not None else seq_linear(0.0, 10.0, 350.0)
+    tg = target_grid(45.0, -45.0, 0.5, n_targets[0], n_targets[1], 100.0, 2000.0) if targets else None
+    return scenario("config1", ocean, freq, pos, de, az, 0.1, tg, loop_steps(0.1, time_max))
+
+
+def _bumps(rng, n, shape_fn):
+    total = None
+    for _ in range(n):
+        term = shape_fn(rng)
+        total = term if total is None else total + term
+    return total
+
+
+def config2_environment(seed=2024, box_deg=10.0, lat0=45.0, lng0=-45.0, bathy_step_deg=1.0 / 60.0):
+    """Seeded gridded ocean of config 2 (see module docstring)."""
+    set_earth_radius(6371000.0)
+    R = earth_radius()
+    rng = np.random.default_rng(seed)
+    # sound speed grid: rho (R - depth) descending, colatitude, longitude
+    n_z, step = 40, 0.25
+    n_h = int(round(box_deg / step)) + 1
+    depths = np.array([5000.0 * k / (n_z - 1) for k in range(n_z)])
+    rho_axis = seq_linear(R, -5000.0 / (n_z - 1), n_z)
+    lat_top = lat0 + box_deg / 2
+    th_axis = seq_linear(to_colatitude(lat_top), to_radians(step), n_h)
+    ph_axis = seq_linear(to_radians(lng0 - box_deg / 2), to_radians(step), n_h)
+    lat = lat_top - step * np.arange(n_h)
+    lng = (lng0 - box_deg / 2) + step * np.arange(n_h)
+    Z, LA, LO = np.meshgrid(depths, lat, lng, indexing="ij")
+
+    def bump(r):
+        la = r.uniform(lat0 - box_deg / 2, lat0 + box_deg / 2)
+        lo = r.uniform(lng0 - box_deg / 2, lng0 + box_deg / 2)
+        z = r.uniform(0.0, 5000.0)
+        sh = r.uniform(0.5, 2.0)
+        sz = r.uniform(200.0, 1500.0)
+        a = r.normal()
+        return a * np.exp(-((LA - la) ** 2 + (LO - lo) ** 2) / (2 * sh * sh)
+                          - (Z - z) ** 2 / (2 * sz * sz))
+
+    dT = _bumps(rng, 64, bump)
+    dT *= 0.5 / math.sqrt(float(np.mean(dT * dT)))
+    dS = _bumps(rng, 64, bump)
+    dS *= 0.2 / math.sqrt(float(np.mean(dS * dS)))
+    T = 25.0 * np.exp(-Z / 800.0) + 2.0 + dT
+    S = 35.0 + dS
+    interp = [oc.PCHIP, oc.LINEAR, oc.LINEAR]
+    tgrid = oc.data_grid([rho_axis, th_axis, ph_axis], T, interp, [True, True, True])
+    sgrid = oc.data_grid([rho_axis, th_axis, ph_axis], S, interp, [True, True, True])
+    ssp = oc.data_grid_mackenzie(tgrid, sgrid)
+    profile = oc.profile_grid(ssp, oc.attenuation_thorp())
+    # bathymetry: 1/60 deg, 50 seamounts / troughs on a 4000 m base
+    n_b = int(round(box_deg / bathy_step_deg)) + 1
+    blat = lat_top - bathy_step_deg * np.arange(n_b)
+    blng = (lng0 - box_deg / 2) + bathy_step_deg * np.arange(n_b)
+    BLA, BLO = np.meshgrid(blat, blng, indexing="ij")
+    depth = np.full(BLA.shape, 4000.0)
+    for _ in range(50):
+        la = rng.uniform(lat0 - box_deg / 2, lat0 + box_deg / 2)
+        lo = rng.uniform(lng0 - box_deg / 2, lng0 + box_deg / 2)
+        w = rng.uniform(0.05, 0.5)
+        amp = rng.uniform(500.0, 2500.0) * (1.0 if rng.uniform() < 0.5 else -1.0)
+        depth -= amp * np.exp(-((BLA - la) ** 2 + (BLO - lo) ** 2) / (2 * w * w))
+    depth = np.clip(depth, 1500.0, 5500.0)
+    bth = seq_linear(to_colatitude(lat_top), to_radians(bathy_step_deg), n_b)
+    bph = seq_linear(to_radians(lng0 - box_deg / 2), to_radians(bathy_step_deg), n_b)
+    bgrid = oc.data_grid([bth, bph], R - depth, [oc.LINEAR, oc.LINEAR], [True, True])
+    bottom = oc.boundary_grid(bgrid, oc.reflect_loss_rayleigh("sand"))
+    surface = oc.boundary_flat()
+    return oc.ocean_model(surface, bottom, profile)
+
+
+def config2(de=None, az=None, n_targets=(100, 100), time_max=120.0, freq=None, seed=2024,
+            ocean=None):
+    ocean = ocean if ocean is not None else config2_environment(seed)
+    freq = freq if freq is not None else seq_log(100.0, 10.0 ** (2.0 / 7.0), 8)
+    pos = wposition1(45.0, -45.0, -1000.0)
+    de = de if de is not None else seq_linear(-80.0, 0.25, 80.0)
+    az = az if az is not None else seq_linear(0.0, 2.0, 358.0)
+    tg = (target_grid(45.0, -45.0, 0.5, n_targets[0], n_targets[1], 100.0, 2000.0)
+          if n_targets else None)
+    return scenario("config2", ocean, freq, pos, de, az, 0.1, tg, loop_steps(0.1, time_max))
+
+
+def eigenray_basic():
+    """waveq3d/test/eigenray_test.cc:86-198 scenario."""
+    compute_earth_radius(45.0)
+    bottom = oc.boundary_flat(3000.0)
+    surface = oc.boundary_flat()
+    profile = oc.profile_linear(1500.0, oc.attenuation_constant(0.0))
+    ocean = oc.ocean_model(surface, bottom, profile)
+    freq = seq_log(10e3, 2.0, 3)
+    pos = wposition1(45.0, -45.0, -1000.0)
+    de = seq_linear(-60.0, 5.0, 60.0)
+    az = seq_linear(-4.0, 1.0, 4.0)
+    target = wposition(1, 1, 45.02, -45.0, -1000.0)
+    return scenario("eigenray_basic", ocean, freq, pos, de, az, 0.1, target, loop_steps(0.1, 3.5))
