@@ -49,6 +49,7 @@ def load():
     lib.orc_take_eigenverbs.argtypes = [vp, C.POINTER(_abi.EigenverbRec), dp]
     lib.orc_take_reflections.argtypes = [vp, C.POINTER(_abi.ReflectionRec)]
     lib.orc_snapshot.argtypes = [vp, C.c_int32, C.POINTER(_abi.FrontView)]
+    lib.orc_math.argtypes = [C.c_int32, dp, dp, dp, i64]
     lib.orc_grid_interp.restype = C.c_double
     lib.orc_grid_interp.argtypes = [C.POINTER(_abi.Grid), dp, dp]
     lib.orc_sound_speed.argtypes = [C.POINTER(_abi.ProfileDesc), C.c_double, C.c_double,
@@ -209,4 +210,17 @@ def height(boundary, rho, theta, phi):
     d = boundary.to_abi()
     out = np.zeros(4)
     lib.orc_height(C.byref(d), rho, theta, phi, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def math_fn(fn, x, y=None):
+    """Shared elementary function `fn` (0 sin, 1 cos, 2 exp, 3 log, 4 log10,
+    5 atan2, 6 asin, 7 10^x) evaluated on the host."""
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(x if y is None else y, dtype=np.float64)
+    out = np.empty_like(x)
+    dp = C.POINTER(C.c_double)
+    lib.orc_math(int(fn), x.ctypes.data_as(dp), y.ctypes.data_as(dp), out.ctypes.data_as(dp),
+                 x.size)
     return out

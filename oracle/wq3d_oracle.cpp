@@ -31,7 +31,6 @@
  */
 #include <algorithm>
 #include <cmath>
-#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -39,6 +38,7 @@
 #include <vector>
 
 #include "../include/wq3d.h"
+#include "../include/wq3d_math.h"
 
 namespace orc {
 
@@ -84,7 +84,7 @@ struct Seq {
             return (size_t)std::max(0L, std::min(n2, k));
         }
         if (search == WQ3D_SEQ_LOG) {
-            long k = (long)std::floor(std::log(value / v[0]) / std::log(inc[1] / inc[0]));
+            long k = (long)std::floor(wqm::log(value / v[0]) / wqm::log(inc[1] / inc[0]));
             return (size_t)std::max(0L, std::min(n2, k));
         }
         value *= sign;
@@ -340,8 +340,8 @@ struct Profile {
         if (kind == WQ3D_PROFILE_MUNK) {
             const double alt = rho - R;
             const double z = 2 * (-alt - p[0]) / p[1];
-            speed = (((z - 1.0) + std::exp(-z)) * p[3] + 1.0) * p[2];
-            grho = (1.0 - std::exp(-z)) * (-p[3] * p[2] * 2 / p[1]);
+            speed = (((z - 1.0) + wqm::exp(-z)) * p[3] + 1.0) * p[2];
+            grho = (1.0 - wqm::exp(-z)) * (-p[3] * p[2] * 2 / p[1]);
         } else if (kind == WQ3D_PROFILE_LINEAR) {
             const double z = -(rho - R);
             if (z < p[2]) {
@@ -456,7 +456,7 @@ struct Boundary {
             *h = grid.interpolate(loc, n ? g : nullptr);
             if (n) {
                 const double t = -g[0] / (*h);
-                const double p = -g[1] / ((*h) * std::sin(theta));
+                const double p = -g[1] / ((*h) * wqm::sin(theta));
                 const double N = std::sqrt(1 + t * t + p * p);
                 n[0] = 1.0 / N;
                 n[1] = t / N;
@@ -465,13 +465,13 @@ struct Boundary {
         }
     }
 
-    std::complex<double> impedance(double density, double speed, double attenuation,
-                                   double angle, std::complex<double>* cosA, bool shear) const {
-        const std::complex<double> c(speed, -attenuation * speed);
-        std::complex<double> sinA = std::sin(angle) * c / spd_w;
-        *cosA = std::sqrt(1.0 - sinA * sinA);
-        if (shear) return c * density / *cosA;
-        return *cosA / (c * density);
+    wqm::cplx impedance(double density, double speed, double attenuation, double angle,
+                        wqm::cplx* cosA, bool shear) const {
+        const wqm::cplx c = {speed, -attenuation * speed};
+        const wqm::cplx sinA = wqm::cdivs(wqm::cscale(c, wqm::sin(angle)), spd_w);
+        *cosA = wqm::csqrt(wqm::csub({1.0, 0.0}, wqm::cmul(sinA, sinA)));
+        if (shear) return wqm::cdiv(wqm::cscale(c, density), *cosA);
+        return wqm::cdiv(*cosA, wqm::cscale(c, density));
     }
 
     // reflect_loss_constant.h, reflect_loss_rayleigh.cc:83-124
@@ -485,19 +485,22 @@ struct Boundary {
         }
         if (angle >= M_PI_2) angle = M_PI_2 - 1e-10;
         angle = M_PI_2 - angle;
-        const double Zw = std::cos(angle) / (spd_w * den_w);
-        std::complex<double> cosAp, cosAs;
-        std::complex<double> Zb = impedance(den_b, spd_b, att_b, angle, &cosAp, false);
+        const double Zw = wqm::cos(angle) / (spd_w * den_w);
+        wqm::cplx cosAp, cosAs;
+        wqm::cplx Zb = impedance(den_b, spd_b, att_b, angle, &cosAp, false);
         if (spd_s != 0.0 || att_s != 0.0) {
-            const std::complex<double> Zs = impedance(den_b, spd_s, att_s, angle, &cosAs, true);
-            const std::complex<double> sinAs = std::sqrt(1.0 - cosAs * cosAs);
-            const std::complex<double> cos2As = 2.0 * cosAs * cosAs - 1.0;
-            const std::complex<double> sin2As = 2.0 * sinAs * cosAs;
-            Zb = 1.0 / ((1.0 / Zb) * cos2As * cos2As + Zs * sin2As * sin2As);
+            const wqm::cplx one = {1.0, 0.0};
+            const wqm::cplx Zs = impedance(den_b, spd_s, att_s, angle, &cosAs, true);
+            const wqm::cplx sinAs = wqm::csqrt(wqm::csub(one, wqm::cmul(cosAs, cosAs)));
+            const wqm::cplx cos2As = wqm::csub(wqm::cmul(wqm::cscale(cosAs, 2.0), cosAs), one);
+            const wqm::cplx sin2As = wqm::cmul(wqm::cscale(sinAs, 2.0), cosAs);
+            Zb = wqm::cdiv(one, wqm::cadd(wqm::cmul(wqm::cmul(wqm::cdiv(one, Zb), cos2As), cos2As),
+                                          wqm::cmul(wqm::cmul(Zs, sin2As), sin2As)));
         }
-        const std::complex<double> R = (Zb - Zw) / (Zb + Zw);
-        const double a = -20.0 * std::log10(std::abs(R)) * 1.0;
-        const double ph = std::arg(R) * 1.0;
+        const wqm::cplx Zwc = {Zw, 0.0};
+        const wqm::cplx R = wqm::cdiv(wqm::csub(Zb, Zwc), wqm::cadd(Zb, Zwc));
+        const double a = -20.0 * wqm::log10(wqm::cabs(R)) * 1.0;
+        const double ph = wqm::carg(R) * 1.0;
         for (size_t f = 0; f < nf; ++f) {
             amp[f] = a;
             if (phase) phase[f] = ph;
@@ -598,8 +601,8 @@ struct Queue {
         const double dcc_r = w.cr[i] / w.c[i];
         const double dcc_t = w.ct[i] / w.c[i];
         const double dcc_p = w.cp[i] / w.c[i];
-        const double st = std::sin(w.pt[i]);
-        const double cot = std::cos(w.pt[i]) / st;
+        const double st = wqm::sin(w.pt[i]);
+        const double cot = wqm::cos(w.pt[i]) / st;
         double c2r = w.c[i] * w.c[i];
         w.gr[i] = c2r * w.nr[i];
         c2r = c2r / w.pr[i];
@@ -657,7 +660,7 @@ struct Queue {
         const double dth = dt * (A2 * y2.gt[i2] - A1 * y1.gt[i1] + A0 * y0.gt[i0]);
         const double dph = dt * (A2 * y2.gp[i2] - A1 * y1.gp[i1] + A0 * y0.gp[i0]);
         const double a = y2.pr[i2] * dth;
-        const double b = y2.pr[i2] * (std::sin(y2.pt[i2]) * dph);
+        const double b = y2.pr[i2] * (wqm::sin(y2.pt[i2]) * dph);
         y3.dist[i3] = std::sqrt(dr * dr + a * a + b * b);
         y3.pr[i3] = y2.pr[i2] + dr;
         y3.pt[i3] = y2.pt[i2] + dth;
@@ -698,7 +701,7 @@ struct Queue {
             tgt_t.assign(fd.target_theta, fd.target_theta + nt);
             tgt_p.assign(fd.target_phi, fd.target_phi + nt);
             tgt_s.resize(nt);
-            for (size_t n = 0; n < nt; ++n) tgt_s[n] = std::sin(tgt_t[n]);
+            for (size_t n = 0; n < nt; ++n) tgt_s[n] = wqm::sin(tgt_t[n]);
         }
         // source clamp: affects _source_pos only; init_wave gets the
         // constructor's argument (wave_queue.cc:58-68, 83)
@@ -732,11 +735,11 @@ struct Queue {
     void init_wave(double r, double t, double p) {
         Front& w = *curr;
         for (size_t d = 0; d < nde(); ++d) {
-            const double cos_de = std::cos(to_radians(de_seq(d)));
-            const double sin_de = std::sin(to_radians(de_seq(d)));
+            const double cos_de = wqm::cos(to_radians(de_seq(d)));
+            const double sin_de = wqm::sin(to_radians(de_seq(d)));
             for (size_t a = 0; a < naz(); ++a) {
-                const double cos_az = std::cos(to_radians(az_seq(a)));
-                const double sin_az = std::sin(to_radians(az_seq(a)));
+                const double cos_az = wqm::cos(to_radians(az_seq(a)));
+                const double sin_az = wqm::sin(to_radians(az_seq(a)));
                 const size_t i = w.id(d, a);
                 w.nr[i] = sin_de;
                 w.nt[i] = -cos_de * cos_az;
@@ -974,7 +977,7 @@ struct Queue {
         double pos[3], nd[3];
         collision_location(de, az, time_water, pos, nd, &c);
         const double grazing =
-            std::atan2(curr->nr[i], std::sqrt(curr->nt[i] * curr->nt[i] + curr->np[i] * curr->np[i]));
+            wqm::atan2(curr->nr[i], std::sqrt(curr->nt[i] * curr->nt[i] + curr->np[i] * curr->np[i]));
         if (grazing <= 0.0) return false;
         if (want_refl) notify_reflection(de, az, time_water, grazing, c, pos, nd, 1);
         if (want_verbs) build_eigenverb(de, az, time_water, grazing, c, pos, nd, 1);
@@ -1038,7 +1041,7 @@ struct Queue {
         } else if (dot_full / c <= -1.0) {
             grazing = M_PI_2;
         } else {
-            grazing = std::asin(-dot_full / c);
+            grazing = wqm::asin(-dot_full / c);
         }
         if (want_refl) notify_reflection(de, az, time_water, grazing, c, pos, nd, 0);
         if (want_verbs) build_eigenverb(de, az, time_water, grazing, c, pos, nd, 0);
@@ -1137,7 +1140,7 @@ struct Queue {
             const double time_water = (d == 0.0) ? 0.0 : -(curr->pr[i] - R) / d;
             double pos[3], nd[3];
             collision_location(de, az, time_water, pos, nd, &c);
-            const double grazing = std::atan2(
+            const double grazing = wqm::atan2(
                 curr->nr[i], std::sqrt(curr->nt[i] * curr->nt[i] + curr->np[i] * curr->np[i]));
             const int type = (d1 > 0) ? 3 + (int)l * 2 : 2 + (int)l * 2;
             build_eigenverb(de, az, time_water, grazing, c, pos, nd, type);
@@ -1159,23 +1162,23 @@ struct Queue {
         const double az_plus = az_angle + 0.5 * to_radians(az_seq.increment(az));
         const size_t az_index = (az == 0 && az_boundary) ? max_az : az;
         const double az_minus = az_angle - 0.5 * to_radians(az_seq.increment(az_index - 1));
-        const double area = (std::sin(de_plus) - std::sin(de_minus)) * (az_plus - az_minus);
+        const double area = (wqm::sin(de_plus) - wqm::sin(de_minus)) * (az_plus - az_minus);
         const double de_delta = de_plus - de_minus;
         const double az_delta = area / de_delta;
         const double path_length = curr->path[i] + curr->c[i] * dt;
         if (path_length <= 0) return;
-        const double sin_grazing = std::sin(grazing);
+        const double sin_grazing = wqm::sin(grazing);
         wq3d_eigenverb v{};
         v.length = 0.5 * path_length * de_delta / sin_grazing;
         v.width = 0.5 * path_length * az_delta;
         std::vector<double> power(nf());
         bool above = false;
         for (size_t f = 0; f < nf(); ++f) {
-            power[f] = std::pow(10.0, -0.1 * curr->atten[i * nf() + f]) * area / sin_grazing;
+            power[f] = wqm::pow10(-0.1 * curr->atten[i * nf() + f]) * area / sin_grazing;
             if (power[f] >= thr.eigenverb) above = true;
         }
         if (!above) return;
-        const double a = to_degrees(std::atan2(nd[2], -nd[1]));
+        const double a = to_degrees(wqm::atan2(nd[2], -nd[1]));
         v.direction = to_radians(a);
         v.step = step_count;
         v.de = (int32_t)de;
@@ -1428,11 +1431,11 @@ struct Queue {
                 double nxt_dir[3], tgt_dir[3];
                 nxt_dir[0] = nxt[0] - cpa[0];
                 nxt_dir[1] = (nxt[1] - prv[1]) * cpa[0];
-                nxt_dir[2] = (nxt[2] - prv[2]) * cpa[0] * std::sin(cpa[1]);
+                nxt_dir[2] = (nxt[2] - prv[2]) * cpa[0] * wqm::sin(cpa[1]);
                 if (std::fabs(nxt_dir[0]) < 0.01) nxt_dir[0] = 0.0;
                 tgt_dir[0] = tgt_r[t] - cpa[0];
                 tgt_dir[1] = (tgt_t[t] - cpa[1]) * cpa[0];
-                tgt_dir[2] = (tgt_p[t] - cpa[2]) * cpa[0] * std::sin(cpa[1]);
+                tgt_dir[2] = (tgt_p[t] - cpa[2]) * cpa[0] * wqm::sin(cpa[1]);
                 const double dot = inner3(nxt_dir, tgt_dir);
                 if (dot < 0.0) {
                     distance[1] *= -1.0;
@@ -1444,8 +1447,8 @@ struct Queue {
 
     // wvector1::direction(double*, double*) (types/wvector1.cc:79-82)
     static void direction(double nr, double nt, double np, double* de, double* az) {
-        *de = to_degrees(std::atan2(nr, std::sqrt(nt * nt + np * np)));
-        *az = to_degrees(std::atan2(np, -nt));
+        *de = to_degrees(wqm::atan2(nr, std::sqrt(nt * nt + np * np)));
+        *az = to_degrees(wqm::atan2(np, -nt));
     }
 
     // wave_queue::build_eigenray (wave_queue.cc:495-702)
@@ -1472,7 +1475,7 @@ struct Queue {
         r.lower = curr->lwr[ic];
         std::vector<double> inten(nf()), spread(nf());
         spreading.intensity(tgt_r[t], tgt_t[t], tgt_p[t], de, az, offset, distance, spread.data());
-        for (size_t f = 0; f < nf(); ++f) inten[f] = -10.0 * std::log10(smax(spread[f], 1e-30));
+        for (size_t f = 0; f < nf(); ++f) inten[f] = -10.0 * wqm::log10(smax(spread[f], 1e-30));
         double dt = offset[0] / time_step;
         if (dt >= 0.0) {
             for (size_t f = 0; f < nf(); ++f)
@@ -1538,7 +1541,7 @@ void Spreading::init(Queue* q) {
         for (size_t a = 0; a < naz - 1; ++a) {
             const double az1 = to_radians(q->az_seq(a));
             const double az2 = to_radians(q->az_seq(a + 1));
-            init_area[d * naz + a] = (std::sin(de2) - std::sin(de1)) * (az2 - az1);
+            init_area[d * naz + a] = (wqm::sin(de2) - wqm::sin(de1)) * (az2 - az1);
             norm_az[d * naz + a] = init_area[d * naz + a] / norm_de[d];
         }
         norm_az[d * naz + naz - 1] = norm_az[d * naz + naz - 2];
@@ -1556,7 +1559,7 @@ void Spreading::init(Queue* q) {
 void Spreading::gaussian(double d, double wdt, double A, double* out) {
     for (size_t f = 0; f < spread.size(); ++f) {
         bw[f] = spread[f] + OVERLAP * OVERLAP * wdt * wdt;
-        out[f] = std::exp((-0.5 * d * d) / bw[f]) / std::sqrt(bw[f]) * A;
+        out[f] = wqm::exp((-0.5 * d * d) / bw[f]) / std::sqrt(bw[f]) * A;
     }
 }
 
@@ -1727,10 +1730,10 @@ void Spreading::intensity_az(size_t de, size_t az, const double offset[3],
 static double ray_distance(const Front& w, size_t i, size_t j) {
     const double r1 = w.pr[i], t1 = w.pt[i], p1 = w.pp[i];
     const double r2 = w.pr[j], t2 = w.pt[j], p2 = w.pp[j];
-    const double sin_theta = std::sin(0.5 * (t1 - t2));
-    const double sin_phi = std::sin(0.5 * (p1 - p2));
+    const double sin_theta = wqm::sin(0.5 * (t1 - t2));
+    const double sin_phi = wqm::sin(0.5 * (p1 - p2));
     const double dotnorm =
-        1.0 - 2.0 * (sin_theta * sin_theta + std::sin(t1) * std::sin(t2) * sin_phi * sin_phi);
+        1.0 - 2.0 * (sin_theta * sin_theta + wqm::sin(t1) * wqm::sin(t2) * sin_phi * sin_phi);
     const double dot = r1 * r2 * dotnorm;
     return std::sqrt(std::fabs(r1 * r1 + r2 * r2 - 2.0 * dot));
 }
@@ -1907,6 +1910,23 @@ void orc_snapshot(void* h, int32_t slot, wq3d_front_view* view) {
     copy_ivec(w->upr, view->upper);
     copy_ivec(w->lwr, view->lower);
     if (view->on_edge) std::copy(w->edge.begin(), w->edge.end(), view->on_edge);
+}
+
+/* Shared elementary functions applied element-wise (accuracy tests):
+ * fn 0 sin, 1 cos, 2 exp, 3 log, 4 log10, 5 atan2(x, y), 6 asin, 7 10^x. */
+void orc_math(int32_t fn, const double* x, const double* y, double* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: out[i] = wqm::sin(x[i]); break;
+            case 1: out[i] = wqm::cos(x[i]); break;
+            case 2: out[i] = wqm::exp(x[i]); break;
+            case 3: out[i] = wqm::log(x[i]); break;
+            case 4: out[i] = wqm::log10(x[i]); break;
+            case 5: out[i] = wqm::atan2(x[i], y[i]); break;
+            case 6: out[i] = wqm::asin(x[i]); break;
+            default: out[i] = wqm::pow10(x[i]); break;
+        }
+    }
 }
 
 /* Interpolate a grid descriptor at one location (pins gen_grid). */

@@ -1,0 +1,77 @@
+"""C ABI checks that need no GPU: libwq3d.so loads, exports every entry
+point include/wq3d.h declares, the ctypes mirrors match the C struct
+layouts, and calls without a device fail with a status instead of crashing."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from underseamodelinglibrary_amd import _abi, _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "wq3d.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const char\*|int32_t|wq3d_status)\s+(wq3d_\w+)\s*\(",
+                                 text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 18
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.library_path()], check=True,
+                         capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\sT\s(wq3d_\w+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        getattr(lib, n)
+
+
+STRUCTS = {
+    "wq3d_seq": _abi.Seq, "wq3d_grid": _abi.Grid, "wq3d_profile_desc": _abi.ProfileDesc,
+    "wq3d_boundary_desc": _abi.BoundaryDesc, "wq3d_volume_desc": _abi.VolumeDesc,
+    "wq3d_env_desc": _abi.EnvDesc, "wq3d_thresholds": _abi.Thresholds,
+    "wq3d_fan_desc": _abi.FanDesc, "wq3d_eigenray": _abi.EigenrayRec,
+    "wq3d_eigenverb": _abi.EigenverbRec, "wq3d_reflection": _abi.ReflectionRec,
+    "wq3d_front_view": _abi.FrontView,
+}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "wq3d.h"', "int main(void){"]
+    for cname, py in STRUCTS.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{cname}.{field} %zu\\n", offsetof({cname}, {field}));')
+    lines.append("return 0;}")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = dict(line.rsplit(" ", 1) for line in out.strip().splitlines())
+    for cname, py in STRUCTS.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert int(got[f"{cname}.{field}"]) == getattr(py, field).offset, f"{cname}.{field}"
+
+
+def test_calls_fail_cleanly_without_device():
+    lib = _lib.load()
+    if lib.wq3d_device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    env = C.c_void_p()
+    desc = _abi.EnvDesc()
+    st = lib.wq3d_env_create(C.byref(desc), C.byref(env))
+    assert st == _abi.WQ3D_ERR_DEVICE
+    assert b"device" in lib.wq3d_last_error()
+    n = C.c_int64()
+    assert lib.wq3d_step(None, 1) == _abi.WQ3D_ERR_ARGUMENT
+    assert lib.wq3d_drain_eigenrays(None, None, None, None, 0, C.byref(n)) == _abi.WQ3D_ERR_ARGUMENT

@@ -42,7 +42,7 @@ struct Arrays {
     // [4*N] per field, slot s at offset s*N
     double *pr, *pt, *pp, *gr, *gt, *gp, *nr, *nt, *np, *hr, *ht, *hp;
     double *c, *cr, *ct, *cp, *dist, *path;
-    double *tr, *tt, *tp, *ts;  // position and sin(theta) at the slot's last update()
+    double *tr, *tt, *tp, *ts;  // position and wqm::sin(theta) at the slot's last update()
     double *atten, *phase;      // [4][nf][N]
     int32_t *srf, *btm, *cst, *upr, *lwr;
     uint8_t* edge;
@@ -149,7 +149,7 @@ __device__ __forceinline__ uint8_t reflect_kind(const DEnv& E, const FanConst& F
                                                 double next_t, double next_p, double cnr, double cnt,
                                                 double cnp) {
     if (next_r - F.R > 0.0) {
-        const double grazing = atan2(cnr, sqrt(cnt * cnt + cnp * cnp));
+        const double grazing = wqm::atan2(cnr, sqrt(cnt * cnt + cnp * cnp));
         if (grazing > 0.0) return 1;
     }
     double h, n[3];
@@ -301,17 +301,17 @@ __device__ void build_eigenverb(RayCtx& x, double dt, double grazing, double spe
     const double az_plus = az_angle + 0.5 * to_radians(seq_inc(F.az, x.az));
     const int az_index = (x.az == 0 && F.az_boundary) ? max_az : x.az;
     const double az_minus = az_angle - 0.5 * to_radians(seq_inc(F.az, (int64_t)az_index - 1));
-    const double area = (sin(de_plus) - sin(de_minus)) * (az_plus - az_minus);
+    const double area = (wqm::sin(de_plus) - wqm::sin(de_minus)) * (az_plus - az_minus);
     const double de_delta = de_plus - de_minus;
     const double az_delta = area / de_delta;
     const double path_length = A.path[kc] + A.c[kc] * dt;
     if (path_length <= 0) return;
-    const double sin_grazing = sin(grazing);
+    const double sin_grazing = wqm::sin(grazing);
     double power[kMaxFreq];
     bool above = false;
     for (int f = 0; f < F.nf; ++f) {
         const double att = A.atten[((int64_t)x.S.curr * F.nf + f) * F.N + x.i];
-        power[f] = pow(10.0, -0.1 * att) * area / sin_grazing;
+        power[f] = wqm::pow10(-0.1 * att) * area / sin_grazing;
         if (power[f] >= F.thr_eigenverb) above = true;
     }
     if (!above) return;
@@ -329,7 +329,7 @@ __device__ void build_eigenverb(RayCtx& x, double dt, double grazing, double spe
     v.type = type;
     v.length = 0.5 * path_length * de_delta / sin_grazing;
     v.width = 0.5 * path_length * az_delta;
-    v.direction = to_radians(to_degrees(atan2(nd[2], -nd[1])));
+    v.direction = to_radians(to_degrees(wqm::atan2(nd[2], -nd[1])));
     v.travel_time = x.time + dt;
     v.grazing = grazing;
     v.sound_speed = speed;
@@ -363,7 +363,7 @@ __device__ void volume_scattering(RayCtx& x) {
         const double time_water = (d == 0.0) ? 0.0 : -(A.pr[kc] - F.R) / d;
         double pos[3], nd[3];
         collision_location(x, x.i, time_water, pos, nd, c);
-        const double grazing = atan2(A.nr[kc], sqrt(A.nt[kc] * A.nt[kc] + A.np[kc] * A.np[kc]));
+        const double grazing = wqm::atan2(A.nr[kc], sqrt(A.nt[kc] * A.nt[kc] + A.np[kc] * A.np[kc]));
         const int type = (d1 > 0) ? 3 + l * 2 : 2 + l * 2;
         build_eigenverb(x, time_water, grazing, c, pos, nd, type);
     }
@@ -423,7 +423,7 @@ __device__ bool surface_reflection(RayCtx& x) {
     const double time_water = (d == 0.0) ? 0.0 : -(A.pr[kc] - F.R) / d;
     double pos[3], nd[3];
     collision_location(x, x.i, time_water, pos, nd, c);
-    const double grazing = atan2(A.nr[kc], sqrt(A.nt[kc] * A.nt[kc] + A.np[kc] * A.np[kc]));
+    const double grazing = wqm::atan2(A.nr[kc], sqrt(A.nt[kc] * A.nt[kc] + A.np[kc] * A.np[kc]));
     if (grazing <= 0.0) return false;
     if (F.want_refl) emit_reflection(x, time_water, grazing, c, pos, nd, 1);
     if (F.want_verbs) build_eigenverb(x, time_water, grazing, c, pos, nd, 1);
@@ -490,7 +490,7 @@ __device__ bool bottom_reflection(RayCtx& x, double depth) {
     } else if (dot_full / c <= -1.0) {
         grazing = M_PI_2;
     } else {
-        grazing = asin(-dot_full / c);
+        grazing = wqm::asin(-dot_full / c);
     }
     if (F.want_refl) emit_reflection(x, time_water, grazing, c, pos, nd, 0);
     if (F.want_verbs) build_eigenverb(x, time_water, grazing, c, pos, nd, 0);
@@ -956,11 +956,11 @@ __device__ void compute_offsets(const BuildCtx& b, int64_t t, int de, int az, co
             double nxt_dir[3], tgt_dir[3];
             nxt_dir[0] = nxt[0] - cpa[0];
             nxt_dir[1] = (nxt[1] - prv[1]) * cpa[0];
-            nxt_dir[2] = (nxt[2] - prv[2]) * cpa[0] * sin(cpa[1]);
+            nxt_dir[2] = (nxt[2] - prv[2]) * cpa[0] * wqm::sin(cpa[1]);
             if (fabs(nxt_dir[0]) < 0.01) nxt_dir[0] = 0.0;
             tgt_dir[0] = F.tgt_r[t] - cpa[0];
             tgt_dir[1] = (F.tgt_t[t] - cpa[1]) * cpa[0];
-            tgt_dir[2] = (F.tgt_p[t] - cpa[2]) * cpa[0] * sin(cpa[1]);
+            tgt_dir[2] = (F.tgt_p[t] - cpa[2]) * cpa[0] * wqm::sin(cpa[1]);
             const double dot = inner3(nxt_dir, tgt_dir);
             if (dot < 0.0) {
                 distance[1] *= -1.0;
@@ -1048,7 +1048,7 @@ __device__ __forceinline__ void gaussian_add(const Spread& s, double d, double w
                                              double* out, bool assign) {
     for (int f = 0; f < s.nf; ++f) {
         const double bw = s.spread[f] + 2.0 * 2.0 * w * w;
-        const double g = exp((-0.5 * d * d) / bw) / sqrt(bw) * A;
+        const double g = wqm::exp((-0.5 * d * d) / bw) / sqrt(bw) * A;
         out[f] = assign ? g : out[f] + g;
     }
 }
@@ -1193,8 +1193,8 @@ __device__ void intensity_az(Spread& s, int de, int az, const double offset[3],
 // wvector1::direction(double*, double*) (types/wvector1.cc:79-82)
 __device__ __forceinline__ void ndir_angles(const Arrays& A, int64_t k, double& de, double& az) {
     const double nr = A.nr[k], nt = A.nt[k], np = A.np[k];
-    de = to_degrees(atan2(nr, sqrt(nt * nt + np * np)));
-    az = to_degrees(atan2(np, -nt));
+    de = to_degrees(wqm::atan2(nr, sqrt(nt * nt + np * np)));
+    az = to_degrees(wqm::atan2(np, -nt));
 }
 
 __global__ void __launch_bounds__(kBlock) k_build(DEnv E, FanConst F, Arrays A, Slots S, Records Rc,
@@ -1247,7 +1247,7 @@ __global__ void __launch_bounds__(kBlock) k_build(DEnv E, FanConst F, Arrays A, 
         }
         double inten[kMaxFreq];
         for (int f = 0; f < F.nf; ++f)
-            inten[f] = -10.0 * log10(smax(sp.I_de[f] * sp.I_az[f], 1e-30));
+            inten[f] = -10.0 * wqm::log10(smax(sp.I_de[f] * sp.I_az[f], 1e-30));
         double dt = offset[0] / F.dt;
         if (dt >= 0.0) {
             for (int f = 0; f < F.nf; ++f) {
@@ -1740,12 +1740,12 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
     {
         std::vector<double> cd(F.nde), sd(F.nde), ca(F.naz), sa(F.naz);
         for (int r = 0; r < F.nde; ++r) {
-            cd[r] = std::cos(to_radians(hde.v[r]));
-            sd[r] = std::sin(to_radians(hde.v[r]));
+            cd[r] = wqm::cos(to_radians(hde.v[r]));
+            sd[r] = wqm::sin(to_radians(hde.v[r]));
         }
         for (int c = 0; c < F.naz; ++c) {
-            ca[c] = std::cos(to_radians(haz.v[c]));
-            sa[c] = std::sin(to_radians(haz.v[c]));
+            ca[c] = wqm::cos(to_radians(haz.v[c]));
+            sa[c] = wqm::sin(to_radians(haz.v[c]));
         }
         if (f->mem.upload(&F.cos_de, cd.data(), cd.size()) != hipSuccess ||
             f->mem.upload(&F.sin_de, sd.data(), sd.size()) != hipSuccess ||
@@ -1753,7 +1753,7 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
             f->mem.upload(&F.sin_az, sa.data(), sa.size()) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "table upload failed"));
     }
-    // targets: sin(theta) cache (wave_queue.cc:54-56) and the _de_branch test
+    // targets: wqm::sin(theta) cache (wave_queue.cc:54-56) and the _de_branch test
     F.nt = nt;
     F.nt2 = (int32_t)std::max<int64_t>(1, d->num_t2);
     f->nt1 = std::max<int64_t>(0, d->num_t1);
@@ -1763,7 +1763,7 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         std::vector<uint8_t> br(nt);
         const double src_lat = 90.0 - to_degrees(f->src_t), src_lng = to_degrees(f->src_p);
         for (int64_t t = 0; t < nt; ++t) {
-            ts[t] = std::sin(d->target_theta[t]);
+            ts[t] = wqm::sin(d->target_theta[t]);
             br[t] = std::fabs(src_lat - (90.0 - to_degrees(d->target_theta[t]))) < 1e-4 &&
                     std::fabs(src_lng - to_degrees(d->target_phi[t])) < 1e-4;
         }
@@ -1788,7 +1788,7 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
                 for (int a = 0; a < naz - 1; ++a) {
                     const double az1 = to_radians(haz.v[a]);
                     const double az2 = to_radians(haz.v[a + 1]);
-                    init_area[(size_t)dd * naz + a] = (std::sin(de2) - std::sin(de1)) * (az2 - az1);
+                    init_area[(size_t)dd * naz + a] = (wqm::sin(de2) - wqm::sin(de1)) * (az2 - az1);
                     naz_v[(size_t)dd * naz + a] = init_area[(size_t)dd * naz + a] / nde_v[dd];
                 }
                 naz_v[(size_t)dd * naz + naz - 1] = naz_v[(size_t)dd * naz + naz - 2];

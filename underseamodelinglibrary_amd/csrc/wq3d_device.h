@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/wq3d_math.h"
+
 namespace wq3d {
 
 constexpr int kMaxFreq = 32;     // frequencies per fan
@@ -82,7 +84,7 @@ __host__ __device__ inline int find_index(const DSeq& s, double value) {
         return (int)(k > n2 ? (n2 < 0 ? 0 : n2) : (k < 0 ? 0 : k));
     }
     if (s.search == 1) {
-        long k = (long)floor(log(value / s.v[0]) / log(s.inc[1] / s.inc[0]));
+        long k = (long)floor(wqm::log(value / s.v[0]) / wqm::log(s.inc[1] / s.inc[0]));
         return (int)(k > n2 ? (n2 < 0 ? 0 : n2) : (k < 0 ? 0 : k));
     }
     value *= s.sign;
@@ -306,8 +308,8 @@ __host__ __device__ inline void sound_speed(const DProfile& P, double rho, doubl
     if (P.kind == 1) {
         const double alt = rho - R;
         const double z = 2 * (-alt - P.p[0]) / P.p[1];
-        speed = (((z - 1.0) + exp(-z)) * P.p[3] + 1.0) * P.p[2];
-        grho = (1.0 - exp(-z)) * (-P.p[3] * P.p[2] * 2 / P.p[1]);
+        speed = (((z - 1.0) + wqm::exp(-z)) * P.p[3] + 1.0) * P.p[2];
+        grho = (1.0 - wqm::exp(-z)) * (-P.p[3] * P.p[2] * 2 / P.p[1]);
     } else if (P.kind == 0) {
         const double z = -(rho - R);
         if (z < P.p[2]) {
@@ -380,7 +382,7 @@ __host__ __device__ inline void boundary_height(const DBoundary& B, double theta
         h = grid_interpolate<2>(B.grid, loc, g, want_normal);
         if (want_normal) {
             const double t = -g[0] / h;
-            const double p = -g[1] / (h * sin(theta));
+            const double p = -g[1] / (h * wqm::sin(theta));
             const double N = sqrt(1 + t * t + p * p);
             n[0] = 1.0 / N;
             n[1] = t / N;
@@ -389,40 +391,20 @@ __host__ __device__ inline void boundary_height(const DBoundary& B, double theta
     }
 }
 
-// minimal complex arithmetic for reflect_loss_rayleigh
-struct cplx {
-    double re, im;
-};
-__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
-    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ cplx cscale(cplx a, double s) { return {a.re * s, a.im * s}; }
-__device__ inline cplx cdiv(cplx a, cplx b) {  // Smith's algorithm
-    if (fabs(b.re) >= fabs(b.im)) {
-        const double r = b.im / b.re;
-        const double d = b.re + b.im * r;
-        return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
-    }
-    const double r = b.re / b.im;
-    const double d = b.im + b.re * r;
-    return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
-}
-__device__ inline cplx csqrt_(cplx z) {  // principal square root
-    if (z.re == 0.0 && z.im == 0.0) return {0.0, z.im};
-    const double t = sqrt((fabs(z.re) + hypot(z.re, z.im)) * 0.5);
-    if (z.re >= 0.0) return {t, z.im / (2.0 * t)};
-    return {fabs(z.im) / (2.0 * t), copysign(t, z.im)};
-}
+using wqm::cadd;
+using wqm::cdiv;
+using wqm::cmul;
+using wqm::cplx;
+using wqm::cscale;
+using wqm::csub;
 
 // reflect_loss_rayleigh::impedance (reflect_loss_rayleigh.cc:129-144)
 __device__ inline cplx rayleigh_impedance(const DBoundary& B, double density, double speed,
                                           double att, double angle, cplx& cosA, bool shear) {
     const cplx c = {speed, -att * speed};
-    const cplx sinA = cscale(c, sin(angle));
+    const cplx sinA = cscale(c, wqm::sin(angle));
     const cplx sA = {sinA.re / B.spd_w, sinA.im / B.spd_w};
-    cosA = csqrt_(csub({1.0, 0.0}, cmul(sA, sA)));
+    cosA = wqm::csqrt(csub({1.0, 0.0}, cmul(sA, sA)));
     if (shear) return cdiv(cscale(c, density), cosA);
     return cdiv(cosA, cscale(c, density));
 }
@@ -437,13 +419,13 @@ __device__ inline void reflect_loss(const DBoundary& B, double angle, double& am
     }
     if (angle >= M_PI_2) angle = M_PI_2 - 1e-10;
     angle = M_PI_2 - angle;
-    const double Zw = cos(angle) / (B.spd_w * B.den_w);
+    const double Zw = wqm::cos(angle) / (B.spd_w * B.den_w);
     cplx cosAp, cosAs;
     cplx Zb = rayleigh_impedance(B, B.den_b, B.spd_b, B.att_b, angle, cosAp, false);
     if (B.spd_s != 0.0 || B.att_s != 0.0) {
         const cplx Zs = rayleigh_impedance(B, B.den_b, B.spd_s, B.att_s, angle, cosAs, true);
         const cplx one = {1.0, 0.0};
-        const cplx sinAs = csqrt_(csub(one, cmul(cosAs, cosAs)));
+        const cplx sinAs = wqm::csqrt(csub(one, cmul(cosAs, cosAs)));
         const cplx cos2As = csub(cmul(cscale(cosAs, 2.0), cosAs), one);
         const cplx sin2As = cmul(cscale(sinAs, 2.0), cosAs);
         const cplx t1 = cmul(cmul(cdiv(one, Zb), cos2As), cos2As);
@@ -452,8 +434,8 @@ __device__ inline void reflect_loss(const DBoundary& B, double angle, double& am
     }
     const cplx Zwc = {Zw, 0.0};
     const cplx Rc = cdiv(csub(Zb, Zwc), cadd(Zb, Zwc));
-    amp = -20.0 * log10(hypot(Rc.re, Rc.im)) * 1.0;
-    phase = atan2(Rc.im, Rc.re) * 1.0;
+    amp = -20.0 * wqm::log10(wqm::cabs(Rc)) * 1.0;
+    phase = wqm::carg(Rc) * 1.0;
 }
 
 // --------------------------------------------------------- one ray's state
@@ -477,8 +459,8 @@ __device__ inline void update_ray(const DEnv& E, double R, Ray& w) {
     const double dcc_r = w.cr / w.c;
     const double dcc_t = w.ct / w.c;
     const double dcc_p = w.cp / w.c;
-    const double st = sin(w.pt);
-    const double cot = cos(w.pt) / st;
+    const double st = wqm::sin(w.pt);
+    const double cot = wqm::cos(w.pt) / st;
     double c2r = w.c * w.c;
     w.gr = c2r * w.nr;
     c2r = c2r / w.pr;
@@ -531,7 +513,7 @@ __device__ __forceinline__ void ab3(double dt, const Ray& y0, const Ray& y1, con
     const double dth = dt * (A2 * y2.gt - A1 * y1.gt + A0 * y0.gt);
     const double dph = dt * (A2 * y2.gp - A1 * y1.gp + A0 * y0.gp);
     const double a = y2.pr * dth;
-    const double b = y2.pr * (sin(y2.pt) * dph);
+    const double b = y2.pr * (wqm::sin(y2.pt) * dph);
     y3.dist = sqrt(dr * dr + a * a + b * b);
     y3.pr = y2.pr + dr;
     y3.pt = y2.pt + dth;
@@ -544,9 +526,9 @@ __device__ __forceinline__ void ab3(double dt, const Ray& y0, const Ray& y1, con
 // wvector1::distance between two points (types/wvector1.cc:44-73)
 __device__ __forceinline__ double point_distance(double r1, double t1, double p1, double r2,
                                                  double t2, double p2) {
-    const double sin_theta = sin(0.5 * (t1 - t2));
-    const double sin_phi = sin(0.5 * (p1 - p2));
-    const double dotnorm = 1.0 - 2.0 * (sin_theta * sin_theta + sin(t1) * sin(t2) * sin_phi * sin_phi);
+    const double sin_theta = wqm::sin(0.5 * (t1 - t2));
+    const double sin_phi = wqm::sin(0.5 * (p1 - p2));
+    const double dotnorm = 1.0 - 2.0 * (sin_theta * sin_theta + wqm::sin(t1) * wqm::sin(t2) * sin_phi * sin_phi);
     const double dot = r1 * r2 * dotnorm;
     return sqrt(fabs(r1 * r1 + r2 * r2 - 2.0 * dot));
 }
