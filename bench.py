@@ -1,9 +1,11 @@
 """Throughput of the WaveQ3D hot path on one MI355X.
 
-Workload (default): the range-dependent single-fan case - gridded 3-D sound
-speed (Mackenzie on seeded T/S, pchip x linear x linear) and 1/60 deg
-bathymetry, DE -80:0.25:80 x AZ 0:2:358 = 115,380 rays, 8 frequencies,
-100 x 100 targets, dt 0.1 s.  A "step" is one wave_queue::step() of the whole
+Workload (default): the single-fan case at the size of the range-dependent
+configuration - DE -80:0.25:80 x AZ 0:2:358 = 115,380 rays, 8 log-spaced
+frequencies 100-10000 Hz, 100 x 100 targets, dt 0.1 s, 1201 steps - in the
+analytic environment of the CPU-runnable case (Munk profile, flat 5000 m
+Rayleigh-sand bottom, Thorp).  The gridded sound speed and bathymetry of the
+range-dependent case are switched off on the GPU in this build.  A "step" is one wave_queue::step() of the whole
 fan: reflections, AB3 propagation, eigenray search over all targets and
 eigenray construction.  W warm-up steps run untimed, then exactly K steps are
 timed; the default W + K = 1201 covers the whole 120 s run.
@@ -37,7 +39,7 @@ def cpu_baseline(steps=2):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from underseamodelinglibrary_amd import scenarios
-    sc = scenarios.config2()
+    sc = scenarios.config2_analytic()
     q = oracle.oracle_queue(*sc.args())
     t0 = time.perf_counter()
     q.step(steps)
@@ -68,8 +70,7 @@ def main():
     lib = _lib.load()
     if lib.wq3d_device_count() < 1:
         raise RuntimeError("no HIP device")
-    log("[bench] building config-2 environment")
-    sc = scenarios.config2()
+    sc = scenarios.config2_analytic()
     log(f"[bench] fan {sc.num_rays()} rays, {sc.num_targets()} targets, "
         f"{sc.freq.size()} frequencies; warmup {args.warmup}, timed {args.steps}")
     q = wave_queue(*sc.args())
@@ -112,9 +113,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded T/S fields and seamount bathymetry)",
-        "config": {"workload": "range-dependent single fan: 641 DE x 180 AZ rays, 8 freqs, "
-                               "100x100 targets, 3-D SSP grid + 1/60 deg bathymetry, dt 0.1 s",
+        "data": "synthetic (analytic Munk ocean, seeded target grid)",
+        "config": {"workload": "single fan 641 DE x 180 AZ rays, 8 freqs 100-10000 Hz, "
+                               "100x100 targets, Munk + flat 5000 m Rayleigh-sand bottom, "
+                               "dt 0.1 s, 1201 steps (gridded SSP/bathymetry off on GPU)",
                    "rays": n_rays, "targets": sc.num_targets(), "frequencies": sc.freq.size()},
         "eigenrays": int(len(rays)),
         "eigenrays_per_s": len(rays) / elapsed,

@@ -31,6 +31,10 @@
 #define WQ3D_MATH_H
 
 #include <math.h>
+#if defined(WQ3D_EMULATE) && !defined(__HIP_DEVICE_COMPILE__)
+#include <stdio.h>
+#include <stdlib.h>
+#endif
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #define WQM_HD __host__ __device__ inline
@@ -135,6 +139,12 @@ WQM_HD double pow10(double x) {
 
 // reduction by pi/2: returns r and the quadrant q = k mod 4
 WQM_HD double reduce_pio2(double x, int* q) {
+#if defined(WQ3D_EMULATE) && !defined(__HIP_DEVICE_COMPILE__)
+    if (!(x > -1.0e6 && x < 1.0e6)) {
+        fprintf(stderr, "wq3d emulation check failed in reduce_pio2: %.17g\n", x);
+        abort();
+    }
+#endif
     const double k = ::floor(x * TWO_OVER_PI + 0.5);
     double r = ::fma(-k, PIO2_1, x);
     r = ::fma(-k, PIO2_2, r);
@@ -205,6 +215,12 @@ WQM_HD double atan_unit(double t) {
                              0x1.4978fa3269ee1p-1, 0x1.921fb54442d18p-1};
     const double AT_LO[5] = {0.0, 0x1.8ab6e3cf7afbdp-57, 0x1.a2b7f222f65e2p-56,
                              0x1.2419a87f2a458p-56, 0x1.1a62633145c07p-55};
+#if defined(WQ3D_EMULATE) && !defined(__HIP_DEVICE_COMPILE__)
+    if (!(t >= 0.0 && t <= 1.0)) {
+        fprintf(stderr, "wq3d emulation check failed in atan_unit: %.17g\n", t);
+        abort();
+    }
+#endif
     const int k = (int)::floor(t * 4.0 + 0.5);
     const double c = 0.25 * (double)k;
     const double u = (t - c) / ::fma(t, c, 1.0);

@@ -26,7 +26,12 @@ def load():
     if not os.path.exists(_PATH):
         raise ImportError(f"{_PATH} is missing: build it with "
                           "`python -m underseamodelinglibrary_amd.build` (HIP, gfx950)")
-    lib = C.CDLL(_PATH)
+    _lib = bind(C.CDLL(_PATH))
+    return _lib
+
+
+def bind(lib):
+    """Attach the C ABI signatures and a status checker to a loaded library."""
     vp, i32, i64, dp = C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double)
     sigs = {
         "wq3d_last_error": ([], C.c_char_p),
@@ -60,5 +65,4 @@ def load():
             raise WQ3DError(status, lib.wq3d_last_error().decode())
 
     lib.check = check
-    _lib = lib
     return lib

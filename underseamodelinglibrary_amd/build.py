@@ -23,6 +23,9 @@ FLAGS = [
 ]
 
 
+EMU_LIB = os.path.join(OUT_DIR, "libwq3d_emu.so")
+
+
 def build(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
     tmp = LIB + ".tmp"
@@ -32,6 +35,26 @@ def build(verbose=False):
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB
+
+
+def build_emulation(asan=True, verbose=False):
+    """CPU build of the same kernel bodies (serial loops, host memory) for
+    tests that check every access with AddressSanitizer.  Never loaded by
+    the package itself."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    tmp = EMU_LIB + ".tmp"
+    flags = ["--offload-host-only", "-O1", "-g", "-std=c++17", "-fPIC", "-shared",
+             "-ffp-contract=off", "-fno-fast-math", "-DWQ3D_EMULATE"]
+    if subprocess.run(["grep", "-qw", "fma", "/proc/cpuinfo"]).returncode == 0:
+        flags.append("-mfma")
+    if asan:
+        flags += ["-Xarch_host", "-fsanitize=address", "-fno-omit-frame-pointer"]
+    cmd = [HIPCC, *flags, "-I", os.path.join(ROOT, "include"), "-o", tmp, SRC]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, EMU_LIB)
+    return EMU_LIB
 
 
 if __name__ == "__main__":

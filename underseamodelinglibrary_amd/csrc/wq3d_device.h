@@ -11,6 +11,19 @@
 
 #include "../../include/wq3d_math.h"
 
+#if defined(WQ3D_EMULATE) && !defined(__HIP_DEVICE_COMPILE__)
+#include <cstdio>
+#include <cstdlib>
+// CPU emulation build only: abort on arguments whose float-to-integer
+// conversion is undefined in device code
+inline void wq3d_emulation_check(bool ok, const char* where, double value) {
+    if (!ok) {
+        std::fprintf(stderr, "wq3d emulation check failed in %s: %.17g\n", where, value);
+        std::abort();
+    }
+}
+#endif
+
 namespace wq3d {
 
 constexpr int kMaxFreq = 32;     // frequencies per fan
@@ -78,6 +91,9 @@ __host__ __device__ __forceinline__ double seq_inc(const DSeq& s, int64_t i) { r
 // seq_linear::find_index (types/seq_linear.h:89-94), seq_log (seq_log.h:84),
 // seq_data (seq_data.cc:15-36)
 __host__ __device__ inline int find_index(const DSeq& s, double value) {
+#if defined(WQ3D_EMULATE) && !defined(__HIP_DEVICE_COMPILE__)
+    wq3d_emulation_check(value == value && value > -1e15 && value < 1e15, "find_index", value);
+#endif
     const long n2 = (long)s.n - 2;
     if (s.search == 0) {
         long k = (long)floor((value - s.v[0]) / s.inc[0]);
@@ -344,7 +360,7 @@ __host__ __device__ inline void sound_speed(const DProfile& P, double rho, doubl
 
 // attenuation_thorp.cc:14-36 (coefficient recomputed as the reference does),
 // attenuation_constant.cc:50-62
-__device__ __forceinline__ double attenuation(const DProfile& P, double freq, double alt,
+__host__ __device__ __forceinline__ double attenuation(const DProfile& P, double freq, double alt,
                                               double distance) {
     if (P.atten_kind == 1) return P.atten_coeff * distance * freq;
     double F2 = freq;
@@ -399,7 +415,7 @@ using wqm::cscale;
 using wqm::csub;
 
 // reflect_loss_rayleigh::impedance (reflect_loss_rayleigh.cc:129-144)
-__device__ inline cplx rayleigh_impedance(const DBoundary& B, double density, double speed,
+__host__ __device__ inline cplx rayleigh_impedance(const DBoundary& B, double density, double speed,
                                           double att, double angle, cplx& cosA, bool shear) {
     const cplx c = {speed, -att * speed};
     const cplx sinA = cscale(c, wqm::sin(angle));
@@ -411,7 +427,7 @@ __device__ inline cplx rayleigh_impedance(const DBoundary& B, double density, do
 
 // reflect_loss_constant.h, reflect_loss_rayleigh::reflect_loss (.cc:83-124);
 // the same amplitude and phase for every frequency.
-__device__ inline void reflect_loss(const DBoundary& B, double angle, double& amp, double& phase) {
+__host__ __device__ inline void reflect_loss(const DBoundary& B, double angle, double& amp, double& phase) {
     if (B.loss_kind == 0) {
         amp = B.loss[0];
         phase = B.loss[1];
@@ -450,7 +466,7 @@ struct Ray {
 };
 
 // wave_front::update minus attenuation (wave_front.cc:125-178,241-250)
-__device__ inline void update_ray(const DEnv& E, double R, Ray& w) {
+__host__ __device__ inline void update_ray(const DEnv& E, double R, Ray& w) {
     double g[3];
     sound_speed(E.profile, w.pr, w.pt, w.pp, R, w.c, g, true);
     w.cr = g[0];
@@ -473,7 +489,7 @@ __device__ inline void update_ray(const DEnv& E, double R, Ray& w) {
 }
 
 // ode_integ.cc:20-126 (RK3 stages)
-__device__ __forceinline__ void rk1(double dt, const Ray& y0, Ray& y1) {
+__host__ __device__ __forceinline__ void rk1(double dt, const Ray& y0, Ray& y1) {
     y1.pr = y0.pr + 0.5 * dt * y0.gr;
     y1.pt = y0.pt + 0.5 * dt * y0.gt;
     y1.pp = y0.pp + 0.5 * dt * y0.gp;
@@ -481,7 +497,7 @@ __device__ __forceinline__ void rk1(double dt, const Ray& y0, Ray& y1) {
     y1.nt = y0.nt + 0.5 * dt * y0.ht;
     y1.np = y0.np + 0.5 * dt * y0.hp;
 }
-__device__ __forceinline__ void rk2(double dt, const Ray& y0, const Ray& y1, Ray& y2) {
+__host__ __device__ __forceinline__ void rk2(double dt, const Ray& y0, const Ray& y1, Ray& y2) {
     y2.pr = y0.pr + dt * (2.0 * y1.gr - y0.gr);
     y2.pt = y0.pt + dt * (2.0 * y1.gt - y0.gt);
     y2.pp = y0.pp + dt * (2.0 * y1.gp - y0.gp);
@@ -489,7 +505,7 @@ __device__ __forceinline__ void rk2(double dt, const Ray& y0, const Ray& y1, Ray
     y2.nt = y0.nt + dt * (2.0 * y1.ht - y0.ht);
     y2.np = y0.np + dt * (2.0 * y1.hp - y0.hp);
 }
-__device__ __forceinline__ void rk3(double dt, const Ray& y0, const Ray& y1, const Ray& y2,
+__host__ __device__ __forceinline__ void rk3(double dt, const Ray& y0, const Ray& y1, const Ray& y2,
                                     Ray& y3) {
     const double pr = y0.pr + dt / 6.0 * (y0.gr + 4.0 * y1.gr + y2.gr);
     const double pt = y0.pt + dt / 6.0 * (y0.gt + 4.0 * y1.gt + y2.gt);
@@ -506,7 +522,7 @@ __device__ __forceinline__ void rk3(double dt, const Ray& y0, const Ray& y1, con
 }
 
 // ode_integ.cc:131-186 (AB3 position with step length, AB3 ndirection)
-__device__ __forceinline__ void ab3(double dt, const Ray& y0, const Ray& y1, const Ray& y2,
+__host__ __device__ __forceinline__ void ab3(double dt, const Ray& y0, const Ray& y1, const Ray& y2,
                                     Ray& y3) {
     const double A2 = 23.0 / 12.0, A1 = 16.0 / 12.0, A0 = 5.0 / 12.0;
     const double dr = dt * (A2 * y2.gr - A1 * y1.gr + A0 * y0.gr);
@@ -524,7 +540,7 @@ __device__ __forceinline__ void ab3(double dt, const Ray& y0, const Ray& y1, con
 }
 
 // wvector1::distance between two points (types/wvector1.cc:44-73)
-__device__ __forceinline__ double point_distance(double r1, double t1, double p1, double r2,
+__host__ __device__ __forceinline__ double point_distance(double r1, double t1, double p1, double r2,
                                                  double t2, double p2) {
     const double sin_theta = wqm::sin(0.5 * (t1 - t2));
     const double sin_phi = wqm::sin(0.5 * (p1 - p2));
@@ -536,7 +552,7 @@ __device__ __forceinline__ double point_distance(double r1, double t1, double p1
 // wave_front::compute_target_distance (wave_front.cc:222-236) for one pair;
 // hr = 0.5*theta_ray, ht2 = 0.5*theta_target etc. are not pre-scaled so the
 // rounding matches the reference expression exactly.
-__device__ __forceinline__ double target_distance2(double rho, double theta, double phi, double st,
+__host__ __device__ __forceinline__ double target_distance2(double rho, double theta, double phi, double st,
                                                    double Rt, double Tt, double Pt, double St) {
     const double h = 0.5 * (theta - Tt);
     const double g = 0.5 * (phi - Pt);

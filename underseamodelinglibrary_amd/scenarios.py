@@ -171,6 +171,18 @@ def config2(de=None, az=None, n_targets=(100, 100), time_max=120.0, freq=None, s
     return scenario("config2", ocean, freq, pos, de, az, 0.1, tg, loop_steps(0.1, time_max))
 
 
+def config2_analytic(n_targets=(100, 100), time_max=120.0):
+    """Config 2's fan, frequencies, targets and run length (641 x 180 rays,
+    8 log-spaced frequencies 100-10000 Hz, 100 x 100 targets, dt 0.1 s,
+    t < 120 s) in config 1's analytic environment (Munk profile, flat 5000 m
+    Rayleigh-sand bottom, Thorp attenuation)."""
+    sc = config1(de=seq_linear(-80.0, 0.25, 80.0), az=seq_linear(0.0, 2.0, 358.0),
+                 n_targets=n_targets, time_max=time_max)
+    sc.freq = seq_log(100.0, 10.0 ** (2.0 / 7.0), 8)
+    sc.name = "config2_analytic"
+    return sc
+
+
 def eigenray_basic():
     """waveq3d/test/eigenray_test.cc:86-198 scenario."""
     compute_earth_radius(45.0)

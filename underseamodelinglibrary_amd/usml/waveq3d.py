@@ -324,9 +324,9 @@ class wave_queue(queue_base):
     """Device-backed wave_queue; every step runs on the HIP library."""
 
     def __init__(self, ocean, freq, pos, de, az, time_step, target_pos=None,
-                 spreading=_abi.SPREADING_HYBRID_GAUSSIAN, env_handle=None):
+                 spreading=_abi.SPREADING_HYBRID_GAUSSIAN, env_handle=None, lib=None):
         super().__init__(ocean, freq, pos, de, az, time_step, target_pos, spreading)
-        self._lib = _load_lib()
+        self._lib = lib if lib is not None else _load_lib()
         lib = self._lib
         self._own_env = env_handle is None
         self._envh = C.c_void_p()
