@@ -95,3 +95,12 @@ def test_emulated_az_wrap_fan_exact():
         print("eigenrays", run(sc))
     """)
     assert int(out.split("eigenrays")[-1]) > 10
+
+
+def test_emulated_branch_point_exact():
+    """Targets straight above / below the source (the source-branch rule)."""
+    out = _run("""
+        sc = scenarios.eigenray_branch_pt()
+        print("eigenrays", run(sc))
+    """)
+    assert int(out.split("eigenrays")[-1]) >= 36

@@ -158,3 +158,32 @@ def test_source_clamp(oracle_mod):
     assert abs(q.source_pos().altitude() + 0.1) < 1e-9
     q = oracle_mod.oracle_queue(ocean, freq, wposition1(45.0, -45.0, -2000.0), de, az, 0.1)
     assert abs(q.source_pos().altitude() + 1000.0 - 0.1) < 1e-6
+
+
+def test_eigenray_branch_pt(oracle_mod):
+    """waveq3d/test/eigenray_test.cc:428-534: three eigenrays per target,
+    including targets straight above and below the source; ring targets
+    carry the eigenray_basic answers."""
+    sc = scenarios.eigenray_branch_pt()
+    q = oracle_mod.oracle_queue(*sc.args())
+    c = collector()
+    q.add_eigenray_listener(c)
+    while q.time() < 3.5:
+        q.step()
+    per = {}
+    for t1, t2, r in c.rays:
+        per.setdefault(t1, []).append(r)
+    for trg in range(12):
+        rays = per.get(trg, [])
+        assert len(rays) == 3, (trg, len(rays))
+        if trg > 1:
+            d, s, b = rays
+            assert abs(d.intensity[0] - 66.9506) < 0.1
+            assert abs(d.travel_time - 1.484018789) < 0.002
+            assert abs(d.source_de + 0.01) < 0.01 and abs(d.target_de - 0.01) < 0.01
+            assert abs(s.intensity[0] - 69.5211) < 0.1
+            assert abs(s.travel_time - 1.995102731) < 0.002
+            assert abs(s.phase[0] + math.pi) < 1e-6
+            assert abs(s.source_de - 41.93623171) < 0.01
+            assert abs(b.travel_time - 3.051676949) < 0.02
+            assert abs(b.source_de + 60.91257162) < 1.0

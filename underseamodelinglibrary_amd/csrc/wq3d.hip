@@ -60,45 +60,83 @@ __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A
     if (i < F.N) body_advance(i, E, F, A, S);
 }
 
-// Time-CPA pre-test of detect_eigenrays (wave_queue.cc:372-395): thread per
-// interior ray, targets staged through LDS in tiles; pairs that pass are
-// appended to the candidate list with one atomic per wavefront.
+// Eigenray search, first stage (see detect_ray / detect_pair): one thread per
+// interior ray, targets of this block's group staged through LDS in tiles;
+// survivors are compacted per wavefront into an LDS buffer and flushed to
+// the global candidate list with one atomic per block and tile.
+constexpr int kCandBuf = 2048;
+
 __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S, Records Rc,
                                                   int64_t tgt_per_group) {
-    __shared__ double s_r[kTargetTile], s_t[kTargetTile], s_p[kTargetTile], s_s[kTargetTile];
+    __shared__ double s_th[kTargetTile], s_ph[kTargetTile], s_s[kTargetTile], s_R[kTargetTile],
+        s_R2[kTargetTile], s_2R[kTargetTile];
+    __shared__ uint8_t s_br[kTargetTile];
+    __shared__ unsigned long long s_buf[kCandBuf];
+    __shared__ unsigned int s_count;
+    __shared__ unsigned long long s_base;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    RayTriplet r;
+    DetectRay r;
     const bool active = detect_ray(F, A, S, i, r);
     const int64_t t_begin = (int64_t)blockIdx.y * tgt_per_group;
     const int64_t t_end = (F.nt < t_begin + tgt_per_group) ? F.nt : t_begin + tgt_per_group;
     const int lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) s_count = 0;
     for (int64_t t0 = t_begin; t0 < t_end; t0 += kTargetTile) {
         const int tn = (int)((t_end - t0 < (int64_t)kTargetTile) ? t_end - t0 : (int64_t)kTargetTile);
         __syncthreads();
         for (int k = threadIdx.x; k < tn; k += blockDim.x) {
-            s_r[k] = F.tgt_r[t0 + k];
-            s_t[k] = F.tgt_t[t0 + k];
-            s_p[k] = F.tgt_p[t0 + k];
+            s_th[k] = F.tgt_th[t0 + k];
+            s_ph[k] = F.tgt_ph[t0 + k];
             s_s[k] = F.tgt_s[t0 + k];
+            s_R[k] = F.tgt_r[t0 + k];
+            s_R2[k] = F.tgt_r2[t0 + k];
+            s_2R[k] = F.tgt_2r[t0 + k];
+            s_br[k] = F.tgt_branch[t0 + k];
         }
         __syncthreads();
         for (int k = 0; k < tn; ++k) {
-            const bool pass = active && detect_pair(r, s_r[k], s_t[k], s_p[k], s_s[k]);
+            bool pass = false;
+            if (active) {
+                const DTarget t{s_th[k], s_ph[k], s_s[k], s_R[k], s_R2[k], s_2R[k]};
+                pass = detect_pair(r, t, s_br[k] != 0);
+            }
             const unsigned long long mask = __ballot(pass);
             if (mask) {
                 const int leader = __ffsll((long long)mask) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(Rc.cand_count, (unsigned long long)__popcll(mask));
-                base = __shfl(base, leader);
+                unsigned int off = 0;
+                if (lane == leader) off = atomicAdd(&s_count, (unsigned int)__popcll(mask));
+                off = __shfl(off, leader);
                 if (pass) {
-                    const unsigned long long off = base + __popcll(mask & ((1ull << lane) - 1ull));
-                    if ((int64_t)off < Rc.cand_cap)
-                        Rc.cand[off] = ((unsigned long long)(t0 + k) << 32) | (unsigned long long)i;
-                    else
-                        atomicOr(Rc.error, 2);
+                    const unsigned int pos = off + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+                    const unsigned long long key = ((unsigned long long)(t0 + k) << 32) | (unsigned long long)i;
+                    if (pos < (unsigned int)kCandBuf) {
+                        s_buf[pos] = key;
+                    } else {  // LDS buffer full: append directly
+                        const unsigned long long g = atomicAdd(Rc.cand_count, 1ull);
+                        if ((int64_t)g < Rc.cand_cap)
+                            Rc.cand[g] = key;
+                        else
+                            atomicOr(Rc.error, 2);
+                    }
                 }
             }
         }
+        // flush this tile's candidates
+        __syncthreads();
+        const unsigned int n = s_count < (unsigned int)kCandBuf ? s_count : (unsigned int)kCandBuf;
+        if (n > 0) {
+            if (threadIdx.x == 0) s_base = atomicAdd(Rc.cand_count, (unsigned long long)n);
+            __syncthreads();
+            const unsigned long long base = s_base;
+            for (unsigned int k = threadIdx.x; k < n; k += blockDim.x) {
+                if ((int64_t)(base + k) < Rc.cand_cap)
+                    Rc.cand[base + k] = s_buf[k];
+                else
+                    atomicOr(Rc.error, 2);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_count = 0;
     }
 }
 
@@ -219,10 +257,10 @@ void launch_detect(hipStream_t st, unsigned grid, unsigned groups, const FanCons
     hipLaunchKernelGGL(k_detect, dim3(grid, groups), dim3(kBlock), 0, st, F, A, S, R, per_group);
 #else
     for (int64_t i = 0; i < F.N; ++i) {
-        RayTriplet r;
+        DetectRay r;
         if (!detect_ray(F, A, S, i, r)) continue;
         for (int64_t t = 0; t < F.nt; ++t) {
-            if (!detect_pair(r, F.tgt_r[t], F.tgt_t[t], F.tgt_p[t], F.tgt_s[t])) continue;
+            if (!detect_pair(r, dtarget(F, t), F.tgt_branch[t] != 0)) continue;
             const unsigned long long off = (*R.cand_count)++;
             if ((int64_t)off < R.cand_cap)
                 R.cand[off] = ((unsigned long long)t << 32) | (unsigned long long)i;
@@ -700,7 +738,18 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
             br[t] = std::fabs(src_lat - (90.0 - to_degrees(d->target_theta[t]))) < 1e-4 &&
                     std::fabs(src_lng - to_degrees(d->target_phi[t])) < 1e-4;
         }
-        if (f->mem.upload(&F.tgt_r, d->target_rho, nt) != hipSuccess ||
+        std::vector<double> th(nt), ph(nt), r2(nt), tr(nt);
+        for (int64_t t = 0; t < nt; ++t) {
+            th[t] = 0.5 * d->target_theta[t];
+            ph[t] = 0.5 * d->target_phi[t];
+            r2[t] = d->target_rho[t] * d->target_rho[t];
+            tr[t] = 2.0 * d->target_rho[t];
+        }
+        if (f->mem.upload(&F.tgt_th, th.data(), nt) != hipSuccess ||
+            f->mem.upload(&F.tgt_ph, ph.data(), nt) != hipSuccess ||
+            f->mem.upload(&F.tgt_r2, r2.data(), nt) != hipSuccess ||
+            f->mem.upload(&F.tgt_2r, tr.data(), nt) != hipSuccess ||
+            f->mem.upload(&F.tgt_r, d->target_rho, nt) != hipSuccess ||
             f->mem.upload(&F.tgt_t, d->target_theta, nt) != hipSuccess ||
             f->mem.upload(&F.tgt_p, d->target_phi, nt) != hipSuccess ||
             f->mem.upload(&F.tgt_s, ts.data(), nt) != hipSuccess ||
@@ -826,7 +875,7 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
     // targets are split into groups so that the detect grid has enough blocks
     int64_t groups = 1;
     if (F.nt > 0) {
-        groups = std::max<int64_t>(1, (int64_t)2048 / std::max<unsigned>(1u, grid));
+        groups = std::max<int64_t>(1, (int64_t)4096 / std::max<unsigned>(1u, grid));
         groups = std::min<int64_t>(groups, (F.nt + kTargetTile - 1) / kTargetTile);
     }
     const int64_t per_group = F.nt > 0 ? (F.nt + groups - 1) / groups : 0;

@@ -69,6 +69,9 @@ struct FanConst {
     int32_t nt2;
     const double *tgt_r, *tgt_t, *tgt_p, *tgt_s;
     const uint8_t* tgt_branch;  // _de_branch per target (wave_queue.cc:363-368)
+    // per-target terms of the distance expression, each computed once:
+    // theta/2, phi/2, rho^2, 2 rho
+    const double *tgt_th, *tgt_ph, *tgt_r2, *tgt_2r;
     // spreading_hybrid_gaussian norms (spreading_hybrid_gaussian.cc:28-59)
     const double* norm_de;
     const double* norm_az;
@@ -659,34 +662,110 @@ WQ_HD void body_advance(int64_t i, const DEnv& E, const FanConst& F, const Array
 // interior ray, targets staged through LDS in tiles; each pair that passes is
 // appended to the candidate list with one atomic per wavefront.
 
-// Per-ray part of the pre-test: is the ray one detect_eigenrays() visits,
-// and its target-distance snapshots in prev/curr/next.
-struct RayTriplet {
-    double cr, ct, cp, cs, nr, nt, np, ns, pr, pt, pp, ps;
+// ---------------------------------------------------------------- detect
+// The eigenray search of detect_eigenrays (wave_queue.cc:372-398) is split in
+// two: k_detect evaluates, for every (target, interior ray), the time-CPA
+// pre-test and an exact subset of is_closest_ray (the four in-plane
+// neighbours at the current time, with its skip and tie rules); survivors
+// are re-tested in full by k_build.  Dropping a pair here is only ever done
+// on a condition for which is_closest_ray would return false.
+//
+// Distance: the expression of wave_front.cc:227-232 with per-ray and
+// per-target terms hoisted.  It is the same IEEE value: 0.5*(a-b) equals
+// a/2-b/2 (power-of-two scaling commutes with rounding, no underflow at
+// these magnitudes) and 1-2Y is fma(-2,Y,1) because 2Y is exact.
+struct DPoint {
+    double th, ph, r, r2, s;  // theta/2, phi/2, rho, rho^2, sin(theta)
 };
 
+struct DTarget {
+    double th, ph, s, R, R2, twoR;
+};
+
+WQ_HD_INLINE DPoint dpoint(const Arrays& A, int64_t k) {
+    DPoint p;
+    p.th = 0.5 * A.tt[k];
+    p.ph = 0.5 * A.tp[k];
+    p.r = A.tr[k];
+    p.r2 = p.r * p.r;
+    p.s = A.ts[k];
+    return p;
+}
+
+WQ_HD_INLINE double dist2_fast(const DPoint& p, const DTarget& t) {
+    const double h = p.th - t.th;
+    const double g = p.ph - t.ph;
+    return fabs((p.r2 + t.R2) - t.twoR * (p.r * ::fma(-2.0, h * h + t.s * (p.s * (g * g)), 1.0)));
+}
+
+struct DetectRay {
+    DPoint cur, nxt, prv;
+    DPoint nb[4];      // de-1, de+1, az-1, az+1 at the current time
+    bool use[4];       // neighbour not skipped by is_closest_ray (non-branch targets)
+    bool upper[4];     // nde == 2 || naz == 2: rejection on <= instead of <
+};
+
+// Loads one interior, not-on-edge ray (the rays detect_eigenrays visits).
 WQ_HD_INLINE bool detect_ray(const FanConst& F, const Arrays& A, const Slots& S, int64_t i,
-                             RayTriplet& r) {
+                             DetectRay& r) {
     const int64_t N = F.N;
     if (i >= N) return false;
     const int de = (int)(i / F.naz), az = (int)(i % F.naz);
     const int az_start = F.az_boundary ? 0 : 1;
-    if (!(de >= 1 && de < F.nde - 1 && az >= az_start && az < F.naz - 1)) return false;
+    const int max_az = F.naz - 1;
+    if (!(de >= 1 && de < F.nde - 1 && az >= az_start && az < max_az)) return false;
     if (A.edge[at(S.curr, N, i)] != 0) return false;
-    const int64_t kc = at(S.curr, N, i), kn = at(S.next, N, i), kp = at(S.prev, N, i);
-    r.cr = A.tr[kc]; r.ct = A.tt[kc]; r.cp = A.tp[kc]; r.cs = A.ts[kc];
-    r.nr = A.tr[kn]; r.nt = A.tt[kn]; r.np = A.tp[kn]; r.ns = A.ts[kn];
-    r.pr = A.tr[kp]; r.pt = A.tt[kp]; r.pp = A.tp[kp]; r.ps = A.ts[kp];
+    r.cur = dpoint(A, at(S.curr, N, i));
+    r.nxt = dpoint(A, at(S.next, N, i));
+    r.prv = dpoint(A, at(S.prev, N, i));
+    // neighbour indices with the AZ wrap of is_closest_ray (wave_queue.cc:423-431)
+    const int dd[4] = {de - 1, de + 1, de, de};
+    const int nz[4] = {1, 1, 0, 2};
+    const int nd[4] = {0, 2, 1, 1};
+    for (int n = 0; n < 4; ++n) {
+        int a = az + nz[n] - 1;
+        if (F.az_boundary) {
+            if (az + nz[n] == 0) {
+                a = F.naz - 2;
+            } else if (a >= max_az) {
+                a = 0;
+            }
+        }
+        const int64_t j = (int64_t)dd[n] * F.naz + a;
+        bool use = !((a == 0 && !F.az_boundary) || a == max_az);
+        if (use && nd[n] != 1) use = A.edge[at(S.curr, N, j)] == 0;
+        r.use[n] = use;
+        r.upper[n] = nd[n] == 2 || nz[n] == 2;
+        r.nb[n] = dpoint(A, at(S.curr, N, j));
+    }
     return true;
 }
 
-// next > center and prev >= center (wave_queue.cc:384-394)
-WQ_HD_INLINE bool detect_pair(const RayTriplet& r, double Rt, double Tt, double Pt, double St) {
-    const double center = target_distance2(r.cr, r.ct, r.cp, r.cs, Rt, Tt, Pt, St);
-    const double dn = target_distance2(r.nr, r.nt, r.np, r.ns, Rt, Tt, Pt, St);
-    if (!(dn > center)) return false;
-    const double dp = target_distance2(r.pr, r.pt, r.pp, r.ps, Rt, Tt, Pt, St);
-    return !(dp < center);
+// Time-CPA pre-test (next > center, prev >= center; wave_queue.cc:384-394)
+// and, for targets without the source-branch rule, the current-time part of
+// is_closest_ray for the four in-plane neighbours (wave_queue.cc:458-478).
+WQ_HD_INLINE bool detect_pair(const DetectRay& r, const DTarget& t, bool branch) {
+    const double center = dist2_fast(r.cur, t);
+    if (!(dist2_fast(r.nxt, t) > center)) return false;
+    if (dist2_fast(r.prv, t) < center) return false;
+    if (branch) return true;
+    for (int n = 0; n < 4; ++n) {
+        if (!r.use[n]) continue;
+        const double d = dist2_fast(r.nb[n], t);
+        if (r.upper[n] ? (d <= center) : (d < center)) return false;
+    }
+    return true;
+}
+
+WQ_HD_INLINE DTarget dtarget(const FanConst& F, int64_t t) {
+    DTarget x;
+    x.th = F.tgt_th[t];
+    x.ph = F.tgt_ph[t];
+    x.s = F.tgt_s[t];
+    x.R = F.tgt_r[t];
+    x.R2 = F.tgt_r2[t];
+    x.twoR = F.tgt_2r[t];
+    return x;
 }
 
 // ----------------------------------------------------------------- build

@@ -183,6 +183,33 @@ def config2_analytic(n_targets=(100, 100), time_max=120.0):
     return sc
 
 
+def eigenray_branch_pt():
+    """waveq3d/test/eigenray_test.cc:428-534 scenario: AZ fan closing on
+    itself, DE -90:1:90, 12 targets on a 2226 m ring plus targets 500 m
+    above and below the source (the source-branch rule)."""
+    compute_earth_radius(0.0)
+    profile = oc.profile_linear(1500.0, oc.attenuation_constant(0.0))
+    ocean = oc.ocean_model(oc.boundary_flat(), oc.boundary_flat(3000.0), profile)
+    freq = seq_log(1000.0, 1.0, 1)
+    pos = wposition1(0.0, 0.0, -1000.0)
+    de = seq_linear(-90.0, 1.0, 90.0)
+    az = seq_linear(0.0, 15.0, 360.0)
+    n = 12
+    lat = np.zeros((n + 2, 1))
+    lng = np.zeros((n + 2, 1))
+    alt = np.full((n + 2, 1), -1000.0)
+    alt[0, 0] = -1500.0
+    alt[1, 0] = -500.0
+    bearing = 0.0
+    for k in range(2, n + 2):
+        p = wposition1.from_range_bearing(pos, 2226.0, bearing)
+        lat[k, 0], lng[k, 0], alt[k, 0] = p.latitude(), p.longitude(), p.altitude()
+        bearing = bearing + 2.0 * math.pi / n
+    target = wposition.from_arrays(lat, lng, alt)
+    return scenario("eigenray_branch_pt", ocean, freq, pos, de, az, 0.1, target,
+                    loop_steps(0.1, 3.5))
+
+
 def eigenray_basic():
     """waveq3d/test/eigenray_test.cc:86-198 scenario."""
     compute_earth_radius(45.0)

@@ -197,6 +197,20 @@ class wposition1:
         self.rho = float(altitude) + earth_radius()
 
     @classmethod
+    def from_range_bearing(cls, location, range_m, bearing):
+        """Point at a great-circle range (m) and bearing (rad) from location
+        (types/wposition1.cc:23-37)."""
+        lat1 = to_radians(location.latitude())
+        lng1 = to_radians(location.longitude())
+        R = earth_radius() + location.altitude()
+        rng = range_m / R
+        lat2 = math.asin(math.sin(lat1) * math.cos(rng)
+                         + math.cos(lat1) * math.sin(rng) * math.cos(bearing))
+        lng2 = lng1 + math.atan2(math.sin(bearing) * math.sin(rng) * math.cos(lat1),
+                                 math.cos(rng) - math.sin(lat1) * math.sin(lat2))
+        return cls(to_degrees(lat2), to_degrees(lng2), location.altitude())
+
+    @classmethod
     def from_rtp(cls, rho, theta, phi):
         p = cls.__new__(cls)
         p.rho, p.theta, p.phi = float(rho), float(theta), float(phi)
