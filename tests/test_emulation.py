@@ -98,8 +98,11 @@ def test_emulated_az_wrap_fan_exact():
 
 
 def test_emulated_branch_point_exact():
-    """Targets straight above / below the source (the source-branch rule)."""
+    """Targets straight above / below the source (the source-branch rule),
+    with device allocations filled with random bytes instead of 0xff."""
     out = _run("""
+        import os
+        os.environ["WQ3D_EMU_FILL"] = "random"
         sc = scenarios.eigenray_branch_pt()
         print("eigenrays", run(sc))
     """)

@@ -61,6 +61,24 @@ def test_config1_full_parity(oracle_mod):
     assert n > 1000
 
 
+def test_branch_point_parity(oracle_mod):
+    """Targets straight above and below the source, AZ fan closing on itself
+    (0:15:360), DE -90:1:90."""
+    sc = scenarios.eigenray_branch_pt()
+    g, o = _run(sc, oracle_mod)
+    assert _check_all(g, o, sc.freq.size()) >= 36
+
+
+def test_multifrequency_fan_parity(oracle_mod):
+    """The benchmark's frequencies (8 log-spaced, 100-10000 Hz) and
+    environment on a thinned fan and a 20 x 20 target grid over 30 s."""
+    sc = scenarios.config2_analytic(n_targets=(20, 20), time_max=30.0)
+    sc.de = seq_linear(-80.0, 1.0, 80.0)
+    sc.az = seq_linear(0.0, 6.0, 354.0)
+    g, o = _run(sc, oracle_mod)
+    assert _check_all(g, o, sc.freq.size()) > 1000
+
+
 @pytest.mark.skip(reason="gridded environments are disabled on the GPU: their first parity run "
                          "faulted the device and no cause has been shown; the same kernel bodies "
                          "are checked against the oracle in tests/test_emulation.py")

@@ -57,6 +57,26 @@ def build_emulation(asan=True, verbose=False):
     return EMU_LIB
 
 
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "wave_queue_eigenray_basic.cpp")
+CPP_TEST = os.path.join(OUT_DIR, "wave_queue_eigenray_basic")
+
+
+def build_cpp_test(lib=LIB, out=CPP_TEST, extra=(), verbose=False):
+    """The C++ wave_queue mirror's test program, linked against `lib`
+    (libwq3d.so, or the emulation build for CPU runs, which also needs its
+    sanitizer runtime in `extra`)."""
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    lib_dir = os.path.dirname(os.path.abspath(lib))
+    name = os.path.basename(lib)[3:-3]
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-o", out + ".tmp", CPP_TEST_SRC,
+           "-L" + lib_dir, "-l" + name, "-Wl,-rpath," + lib_dir, *extra]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
     sys.exit(0)

@@ -175,11 +175,23 @@ inline hipError_t event_destroy(hipEvent_t e) { return hipEventDestroy(e); }
 inline hipError_t last_error() { return hipGetLastError(); }
 #else
 // device allocations are not zeroed on the GPU: fill with 0xff (NaN doubles,
-// -1 integers) so reads before writes show up in the CPU checks
+// -1 integers) so reads before writes show up in the CPU checks.
+// WQ3D_EMU_FILL=<hex byte> or "random" picks another fill.
 inline hipError_t malloc(void** p, size_t n) {
     *p = std::malloc(n);
-    if (*p) std::memset(*p, 0xff, n);
-    return *p ? hipSuccess : hipErrorOutOfMemory;
+    if (!*p) return hipErrorOutOfMemory;
+    const char* fill = std::getenv("WQ3D_EMU_FILL");
+    if (fill && std::strcmp(fill, "random") == 0) {
+        static unsigned long long state = 0x9e3779b97f4a7c15ull;
+        unsigned char* b = static_cast<unsigned char*>(*p);
+        for (size_t k = 0; k < n; ++k) {
+            state = state * 6364136223846793005ull + 1442695040888963407ull;
+            b[k] = (unsigned char)(state >> 56);
+        }
+    } else {
+        std::memset(*p, fill ? (int)std::strtol(fill, nullptr, 16) : 0xff, n);
+    }
+    return hipSuccess;
 }
 inline hipError_t free(void* p) {
     std::free(p);
