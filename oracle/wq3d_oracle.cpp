@@ -19,15 +19,23 @@
  *     holds after reflection_copy (reflection_model.cc:335-360).
  *   - listeners are replaced by record lists with explicit keys.
  *
- * Pinning (what checks this file; see tests/test_oracle_*.py):
- *   eigenray_basic / eigenray_concave known answers, reflect_flat_test bounce
- *   times, refraction_munk_range cycle ranges, eigenverb closed forms,
- *   Mackenzie / Thorp / Rayleigh textbook values, gen_grid exactness on
- *   linear/cubic data, invariants (|xi| c = 1 after reflection, isovelocity
- *   straight lines).  Parity unpinned: the unstable 2x2 / diagonal branches of
- *   compute_offsets, hybrid-Gaussian walk termination details and pchip
- *   end-point slopes on non-uniform axes, which no reference test constrains
- *   beyond loose tolerances.
+ * Pinning (what checks this file: tests/test_oracle_known_answers.py,
+ * tests/test_oracle_pins.py, tests/test_math.py):
+ *   the reference's known answers for eigenray_basic, eigenray_concave and
+ *   eigenray_branch_pt (travel times, angles, intensities, counts),
+ *   reflect_flat_test bounce times, refraction_munk_range cycle ranges and
+ *   vertex counts, refraction_isovelocity straight lines, eigenverb_accuracy
+ *   lengths / widths / grazing angles, the source clamp, Thorp against the
+ *   Generic Sonar Model table, constant attenuation, gen_grid linear /
+ *   quadratic / derivative accuracy and exact trilinear reproduction; the
+ *   Rayleigh loss against the textbook wavenumber form of the reflection
+ *   coefficient and its critical-angle and normal-incidence limits; the
+ *   shared elementary functions against glibc in ulps.  Parity unpinned:
+ *   the unstable 2x2 / diagonal branches of compute_offsets, the hybrid
+ *   Gaussian walk termination details, pchip end-point slopes on
+ *   non-uniform axes, reflection_reinit numerics beyond bounce times and
+ *   caustic counts on real folds, which no reference test constrains beyond
+ *   loose tolerances.
  */
 #include <algorithm>
 #include <cmath>

@@ -69,6 +69,23 @@ def test_branch_point_parity(oracle_mod):
     assert _check_all(g, o, sc.freq.size()) >= 36
 
 
+def test_eigenray_concave_parity(oracle_mod):
+    """Four eigenrays (direct and three surface paths) after 120 s in a deep
+    isovelocity ocean (the reference's eigenray_concave scenario)."""
+    import math
+    from underseamodelinglibrary_amd.usml.types import compute_earth_radius, set_earth_radius, wposition, wposition1
+    compute_earth_radius(45.0)
+    prof = oc.profile_linear(1500.0, oc.attenuation_constant(0.0))
+    ocean = oc.ocean_model(oc.boundary_flat(), oc.boundary_flat(1e5), prof)
+    sc = scenarios.scenario("eigenray_concave", ocean, seq_log(2000.0, 1.0, 1),
+                            wposition1(45.0, -45.0, -200.0), seq_linear(-1.0, 0.05, 1.0),
+                            seq_linear(-4.0, 1.0, 4.0), 0.1, wposition(1, 1, 46.2, -45.0, -150.0),
+                            scenarios.loop_steps(0.1, 120.0))
+    g, o = _run(sc, oracle_mod)
+    assert _check_all(g, o, 1) == 4
+    set_earth_radius(6371000.0)
+
+
 def test_multifrequency_fan_parity(oracle_mod):
     """The benchmark's frequencies (8 log-spaced, 100-10000 Hz) and
     environment on a thinned fan and a 20 x 20 target grid over 30 s."""
