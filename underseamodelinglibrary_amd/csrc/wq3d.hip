@@ -60,66 +60,126 @@ __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A
     if (i < F.N) body_advance(i, E, F, A, S);
 }
 
-// Eigenray search, first stage (see detect_ray / detect_pair): one thread per
-// interior ray, targets of this block's group staged through LDS in tiles;
-// survivors are compacted per wavefront into an LDS buffer and flushed to
-// the global candidate list with one atomic per block and tile.
-constexpr int kCandBuf = 2048;
+// Eigenray search, first stage (see detect_ray, detect_window_ok,
+// detect_maybe, detect_pair): one thread per interior ray; this block's
+// target group is staged through LDS in tiles (linear-form features and the
+// exact terms of the distance expression).  The main loop runs one linear
+// form and one compare per (ray, target) pair; the few pairs inside the
+// window go to a per-lane queue in LDS, drained once per tile (or when a
+// queue fills) so the exact second stage runs with the lanes of a
+// wavefront in step instead of on divergent branches.  Survivors are
+// compacted per wavefront into an LDS buffer and flushed to the global
+// candidate list with one atomic per block and tile.
+constexpr int kCandBuf = 1024;
+constexpr int kQueue = 16;
+
+struct alignas(16) LinFeatures {
+    double f[6];
+};
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(4))) const LinFeatures ConstLinFeatures;  // constant address space
+#else
+typedef const LinFeatures ConstLinFeatures;
+#endif
+
+__device__ __forceinline__ void append_candidate(bool pass, unsigned long long key, int lane,
+                                                 unsigned int* s_count, unsigned long long* s_buf,
+                                                 const Records& Rc) {
+    const unsigned long long mask = __ballot(pass);
+    if (!mask) return;
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned int off = 0;
+    if (lane == leader) off = atomicAdd(s_count, (unsigned int)__popcll(mask));
+    off = __shfl(off, leader);
+    if (pass) {
+        const unsigned int pos = off + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+        if (pos < (unsigned int)kCandBuf) {
+            s_buf[pos] = key;
+        } else {  // LDS buffer full: append directly
+            const unsigned long long g = atomicAdd(Rc.cand_count, 1ull);
+            if ((int64_t)g < Rc.cand_cap)
+                Rc.cand[g] = key;
+            else
+                atomicOr(Rc.error, 2);
+        }
+    }
+}
 
 __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S, Records Rc,
                                                   int64_t tgt_per_group) {
-    __shared__ double s_th[kTargetTile], s_ph[kTargetTile], s_s[kTargetTile], s_R[kTargetTile],
-        s_R2[kTargetTile], s_2R[kTargetTile];
+    __shared__ LinFeatures s_f[kTargetTile + 1];
+    __shared__ DTarget s_x[kTargetTile];
     __shared__ uint8_t s_br[kTargetTile];
+    __shared__ uint16_t s_q[kQueue][kBlock];
     __shared__ unsigned long long s_buf[kCandBuf];
     __shared__ unsigned int s_count;
     __shared__ unsigned long long s_base;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     DetectRay r;
     const bool active = detect_ray(F, A, S, i, r);
+    const double margin = F.lin_margin;
     const int64_t t_begin = (int64_t)blockIdx.y * tgt_per_group;
     const int64_t t_end = (F.nt < t_begin + tgt_per_group) ? F.nt : t_begin + tgt_per_group;
     const int lane = threadIdx.x & 63;
     if (threadIdx.x == 0) s_count = 0;
+    int qn = 0;
+    // exact second stage for this lane's queued targets of the tile at t0
+    auto drain = [&](int64_t t0) {
+        int maxq = qn;
+        for (int off = 32; off > 0; off >>= 1) maxq = max(maxq, __shfl_xor(maxq, off));
+        for (int j = 0; j < maxq; ++j) {
+            bool pass = false;
+            int k = 0;
+            if (j < qn) {
+                k = s_q[j][threadIdx.x];
+                bool time_sure = false;
+                if (detect_maybe(r, s_f[k].f, margin, time_sure))
+                    pass = detect_pair(r, s_x[k], s_br[k] != 0, time_sure);
+            }
+            append_candidate(pass, ((unsigned long long)(t0 + k) << 32) | (unsigned long long)i, lane,
+                             &s_count, s_buf, Rc);
+        }
+        qn = 0;
+    };
     for (int64_t t0 = t_begin; t0 < t_end; t0 += kTargetTile) {
         const int tn = (int)((t_end - t0 < (int64_t)kTargetTile) ? t_end - t0 : (int64_t)kTargetTile);
         __syncthreads();
-        for (int k = threadIdx.x; k < tn; k += blockDim.x) {
-            s_th[k] = F.tgt_th[t0 + k];
-            s_ph[k] = F.tgt_ph[t0 + k];
-            s_s[k] = F.tgt_s[t0 + k];
-            s_R[k] = F.tgt_r[t0 + k];
-            s_R2[k] = F.tgt_r2[t0 + k];
-            s_2R[k] = F.tgt_2r[t0 + k];
-            s_br[k] = F.tgt_branch[t0 + k];
+        {
+            double* flat = &s_f[0].f[0];
+            const double* src = F.tgt_lin + t0 * 6;
+            for (int k = threadIdx.x; k < tn * 6; k += blockDim.x) flat[k] = src[k];
+            for (int k = threadIdx.x; k < tn; k += blockDim.x) {
+                s_x[k] = dtarget(F, t0 + k);
+                s_br[k] = F.tgt_branch[t0 + k];
+            }
         }
         __syncthreads();
-        for (int k = 0; k < tn; ++k) {
-            bool pass = false;
-            if (active) {
-                const DTarget t{s_th[k], s_ph[k], s_s[k], s_R[k], s_R2[k], s_2R[k]};
-                pass = detect_pair(r, t, s_br[k] != 0);
-            }
-            const unsigned long long mask = __ballot(pass);
-            if (mask) {
-                const int leader = __ffsll((long long)mask) - 1;
-                unsigned int off = 0;
-                if (lane == leader) off = atomicAdd(&s_count, (unsigned int)__popcll(mask));
-                off = __shfl(off, leader);
-                if (pass) {
-                    const unsigned int pos = off + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
-                    const unsigned long long key = ((unsigned long long)(t0 + k) << 32) | (unsigned long long)i;
-                    if (pos < (unsigned int)kCandBuf) {
-                        s_buf[pos] = key;
-                    } else {  // LDS buffer full: append directly
-                        const unsigned long long g = atomicAdd(Rc.cand_count, 1ull);
-                        if ((int64_t)g < Rc.cand_cap)
-                            Rc.cand[g] = key;
-                        else
-                            atomicOr(Rc.error, 2);
-                    }
+        if (__any(active)) {
+            // the target of an iteration is the same for every lane: its
+            // features come through the scalar cache into SGPRs
+            const ConstLinFeatures* cf = (const ConstLinFeatures*)(F.tgt_lin) + t0;
+            auto visit = [&](const LinFeatures& fc, int k) {
+                if (active && detect_window_ok(r, fc.f)) {
+                    s_q[qn][threadIdx.x] = (uint16_t)k;
+                    ++qn;
                 }
+            };
+            int k = 0;
+            // groups of four targets: one batch of scalar loads, at most four
+            // queue entries (the queue is drained while it has room for them)
+            for (; k + 4 <= tn; k += 4) {
+                const LinFeatures f0 = cf[k], f1 = cf[k + 1], f2 = cf[k + 2], f3 = cf[k + 3];
+                visit(f0, k);
+                visit(f1, k + 1);
+                visit(f2, k + 2);
+                visit(f3, k + 3);
+                if (__any(qn > kQueue - 4)) drain(t0);
             }
+            for (; k < tn; ++k) {
+                visit(cf[k], k);
+                if (__any(qn == kQueue)) drain(t0);
+            }
+            drain(t0);
         }
         // flush this tile's candidates
         __syncthreads();
@@ -272,7 +332,10 @@ void launch_detect(hipStream_t st, unsigned grid, unsigned groups, const FanCons
         DetectRay r;
         if (!detect_ray(F, A, S, i, r)) continue;
         for (int64_t t = 0; t < F.nt; ++t) {
-            if (!detect_pair(r, dtarget(F, t), F.tgt_branch[t] != 0)) continue;
+            bool time_sure = false;
+            if (!detect_window_ok(r, F.tgt_lin + t * 6)) continue;
+            if (!detect_maybe(r, F.tgt_lin + t * 6, F.lin_margin, time_sure)) continue;
+            if (!detect_pair(r, dtarget(F, t), F.tgt_branch[t] != 0, time_sure)) continue;
             const unsigned long long off = (*R.cand_count)++;
             if ((int64_t)off < R.cand_cap)
                 R.cand[off] = ((unsigned long long)t << 32) | (unsigned long long)i;
@@ -413,6 +476,24 @@ struct wq3d_fan {
 };
 
 namespace {
+
+// linear-form features of one target (see lin_coeffs)
+void target_features(double R, double at, double bt, double st, double f6[6]) {
+    const double Rs = R * st;
+    f6[0] = R;
+    f6[1] = R * at;
+    f6[2] = R * (at * at);
+    f6[3] = Rs;
+    f6[4] = Rs * bt;
+    f6[5] = Rs * (bt * bt);
+}
+
+// rounding margin of the linear forms for radii up to rmax: 2^-38 * 2 rho^2
+// with 20 km of headroom, at least 64 m^2
+double lin_margin(double rmax) {
+    rmax += 2.0e4;
+    return std::max(64.0, std::ldexp(2.0 * rmax * rmax, -38));
+}
 
 std::once_flag g_dev_once;
 int g_dev_count = 0;
@@ -757,7 +838,26 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
             r2[t] = d->target_rho[t] * d->target_rho[t];
             tr[t] = 2.0 * d->target_rho[t];
         }
-        if (f->mem.upload(&F.tgt_th, th.data(), nt) != hipSuccess ||
+        // linear-form features (see lin_coeffs) and their rounding margin:
+        // 2^-38 * 2 rho^2 at the largest radius in play, at least 64 m^2
+        std::vector<double> lin((size_t)nt * 6);
+        double rmax = F.R;
+        for (int j = 0; j < 6; ++j) {
+            F.lin_fmin[j] = HUGE_VAL;
+            F.lin_fmax[j] = -HUGE_VAL;
+        }
+        for (int64_t t = 0; t < nt; ++t) {
+            double* f6 = &lin[(size_t)t * 6];
+            target_features(d->target_rho[t], th[t], ph[t], ts[t], f6);
+            rmax = std::max(rmax, std::fabs(d->target_rho[t]));
+            for (int j = 0; j < 6; ++j) {
+                F.lin_fmin[j] = std::min(F.lin_fmin[j], f6[j]);
+                F.lin_fmax[j] = std::max(F.lin_fmax[j], f6[j]);
+            }
+        }
+        F.lin_margin = lin_margin(rmax);
+        if (f->mem.upload(&F.tgt_lin, lin.data(), lin.size()) != hipSuccess ||
+            f->mem.upload(&F.tgt_th, th.data(), nt) != hipSuccess ||
             f->mem.upload(&F.tgt_ph, ph.data(), nt) != hipSuccess ||
             f->mem.upload(&F.tgt_r2, r2.data(), nt) != hipSuccess ||
             f->mem.upload(&F.tgt_2r, tr.data(), nt) != hipSuccess ||
@@ -1128,5 +1228,141 @@ wq3d_status wq3d_kernel_times(wq3d_fan* f, double* ms, int64_t* launches, int32_
     }
     return WQ3D_OK;
 }
+
+#ifdef WQ3D_EMULATE
+// Test hook of the CPU emulation build only: random rays and targets near
+// the earth's radius, plus targets placed by bisection on the boundaries
+// where an exact comparison of the pre-test flips, decided once with the
+// linear-form pre-filter and once with the exact expressions alone.
+// counts = {pairs, mismatches, pairs settled by the exact fallback, passes}.
+static double emu_uniform(unsigned long long& s, double lo, double hi) {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    return lo + (hi - lo) * ((double)(s >> 11) * 0x1.0p-53);
+}
+
+static DPoint emu_point(double r, double t, double p) {
+    DPoint q;
+    q.th = 0.5 * t;
+    q.ph = 0.5 * p;
+    q.r = r;
+    q.r2 = r * r;
+    q.s = wqm::sin(t);
+    return q;
+}
+
+int32_t wq3d_emu_check_detect(int64_t num_rays, int64_t num_targets, uint64_t seed, double earth_radius,
+                              int64_t* counts) {
+    unsigned long long s = seed * 2654435761ull + 1;
+    const double R0 = earth_radius;
+    const double deg = M_PI / 180.0;
+    for (int k = 0; k < 4; ++k) counts[k] = 0;
+    const double margin = lin_margin(R0);
+    for (int64_t ray = 0; ray < num_rays; ++ray) {
+        // ray state: position, velocity over one step (about 150 m), curvature
+        const double r = R0 - emu_uniform(s, 1.0, 5500.0);
+        const double t = (90.0 - 45.0 + emu_uniform(s, -1.0, 1.0)) * deg;
+        const double p = (-45.0 + emu_uniform(s, -1.0, 1.0)) * deg;
+        const double ang = emu_uniform(s, 0.0, 2.0 * M_PI), el = emu_uniform(s, -1.2, 1.2);
+        const double vn = 150.0 * std::cos(el) * std::cos(ang), ve = 150.0 * std::cos(el) * std::sin(ang);
+        const double vu = 150.0 * std::sin(el);
+        auto at_step = [&](double k, double bend) {
+            return emu_point(r + k * vu + bend * k * k, t - k * vn / r, p + k * ve / (r * std::sin(t)));
+        };
+        const double bend = emu_uniform(s, -2.0, 2.0);
+        DetectRay dr;
+        dr.cur = at_step(0.0, bend);
+        dr.nxt = at_step(1.0, bend);
+        dr.prv = at_step(-1.0, bend);
+        const int nz[4] = {1, 1, 0, 2};
+        const int nd[4] = {0, 2, 1, 1};
+        for (int n = 0; n < 4; ++n) {
+            dr.nb[n] = emu_point(r + emu_uniform(s, -300.0, 300.0), t + emu_uniform(s, -300.0, 300.0) / r,
+                                 p + emu_uniform(s, -300.0, 300.0) / (r * std::sin(t)));
+            dr.use[n] = emu_uniform(s, 0.0, 1.0) < 0.8;
+            dr.upper[n] = nd[n] == 2 || nz[n] == 2;
+        }
+        dr.next = lin_diff(dr.nxt, dr.cur);
+        dr.prev = lin_diff(dr.prv, dr.cur);
+        auto make = [&](double tr, double tt, double tp, DTarget& x, double f6[6]) {
+            x.th = 0.5 * tt;
+            x.ph = 0.5 * tp;
+            x.s = wqm::sin(tt);
+            x.R = tr;
+            x.R2 = tr * tr;
+            x.twoR = 2.0 * tr;
+            target_features(tr, x.th, x.ph, x.s, f6);
+        };
+        // sign of one exact comparison of the pre-test at a target
+        auto side = [&](int which, double tr, double tt, double tp) {
+            DTarget x;
+            double f6[6];
+            make(tr, tt, tp, x, f6);
+            const double c = dist2_fast(dr.cur, x);
+            const DPoint& q = which == 0 ? dr.nxt : which == 1 ? dr.prv : dr.nb[which - 2];
+            return dist2_fast(q, x) > c;
+        };
+        struct Tgt {
+            double r, t, p;
+            bool branch;
+        };
+        std::vector<Tgt> tg;
+        for (int64_t k = 0; k < num_targets; ++k) {
+            // near the ray: within 2 or 20 km horizontally, 100-2000 m deep
+            const double span = emu_uniform(s, 0.0, 1.0) < 0.5 ? 2000.0 : 20000.0;
+            const double tr = R0 - emu_uniform(s, 100.0, 2000.0);
+            const double tt = t + emu_uniform(s, -span, span) / r;
+            const double tp = p + emu_uniform(s, -span, span) / (r * std::sin(t));
+            const bool branch = emu_uniform(s, 0.0, 1.0) < 0.1;
+            tg.push_back({tr, tt, tp, branch});
+            // a second target, then bisect between the two on each comparison
+            const double ur = R0 - emu_uniform(s, 100.0, 2000.0);
+            const double ut = t + emu_uniform(s, -span, span) / r;
+            const double up = p + emu_uniform(s, -span, span) / (r * std::sin(t));
+            for (int which = 0; which < 6; ++which) {
+                if (side(which, tr, tt, tp) == side(which, ur, ut, up)) continue;
+                double lo = 0.0, hi = 1.0;
+                const bool s0 = side(which, tr, tt, tp);
+                for (int it = 0; it < 80; ++it) {
+                    const double mid = 0.5 * (lo + hi);
+                    const bool sm = side(which, tr + mid * (ur - tr), tt + mid * (ut - tt), tp + mid * (up - tp));
+                    (sm == s0 ? lo : hi) = mid;
+                }
+                for (double w : {lo, hi})
+                    tg.push_back({tr + w * (ur - tr), tt + w * (ut - tt), tp + w * (up - tp), branch});
+            }
+        }
+        FanConst F{};
+        F.lin_margin = margin;
+        for (int j = 0; j < 6; ++j) {
+            F.lin_fmin[j] = HUGE_VAL;
+            F.lin_fmax[j] = -HUGE_VAL;
+        }
+        for (const Tgt& g : tg) {
+            DTarget x;
+            double f6[6];
+            make(g.r, g.t, g.p, x, f6);
+            for (int j = 0; j < 6; ++j) {
+                F.lin_fmin[j] = std::min(F.lin_fmin[j], f6[j]);
+                F.lin_fmax[j] = std::max(F.lin_fmax[j], f6[j]);
+            }
+        }
+        detect_window(F, dr);
+        for (const Tgt& g : tg) {
+            DTarget x;
+            double f6[6];
+            make(g.r, g.t, g.p, x, f6);
+            const bool ref = detect_pair_reference(dr, x, g.branch);
+            bool sure = false, got = false, maybe = false;
+            if (detect_window_ok(dr, f6)) maybe = detect_maybe(dr, f6, margin, sure);
+            if (maybe) got = detect_pair(dr, x, g.branch, sure);
+            counts[0] += 1;
+            counts[1] += ref != got;
+            counts[2] += maybe && !sure;
+            counts[3] += ref;
+        }
+    }
+    return 0;
+}
+#endif
 
 }  // extern "C"

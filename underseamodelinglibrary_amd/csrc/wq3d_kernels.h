@@ -72,6 +72,11 @@ struct FanConst {
     // per-target terms of the distance expression, each computed once:
     // theta/2, phi/2, rho^2, 2 rho
     const double *tgt_th, *tgt_ph, *tgt_r2, *tgt_2r;
+    // [nt][6] target features of the distance expression written as a
+    // linear form (see lin_coeffs) and the rounding margin of that form
+    const double* tgt_lin;
+    double lin_margin;
+    double lin_fmin[6], lin_fmax[6];  // feature ranges over all targets
     // spreading_hybrid_gaussian norms (spreading_hybrid_gaussian.cc:28-59)
     const double* norm_de;
     const double* norm_az;
@@ -698,12 +703,87 @@ WQ_HD_INLINE double dist2_fast(const DPoint& p, const DTarget& t) {
     return fabs((p.r2 + t.R2) - t.twoR * (p.r * ::fma(-2.0, h * h + t.s * (p.s * (g * g)), 1.0)));
 }
 
+// The time-CPA pre-test compares distances of one target to the same ray at
+// three times.  Expanding the distance expression above,
+//   d2 = (r - R)^2 + 4 r R ((a - at)^2 + s st (b - bt)^2)
+// (a = theta/2, b = phi/2, s = sin theta of the ray; at, bt, st, R of the
+// target), a difference d2(q) - d2(p) of two ray points is linear in six
+// target features
+//   F = (R, R at, R at^2, R st, R st bt, R st bt^2)
+// with coefficients c(q) - c(p), c = (-2r + 4r a^2, -8r a, 4r, 4r s b^2,
+// -8r s b, 4r s), plus (r_q - r_p)(r_q + r_p).  Six fused multiply-adds give
+// the sign of the difference whenever its magnitude exceeds lin_margin, a
+// bound on the rounding of the estimate and of the two exact distances
+// (both well under 1 m^2 for positions near the earth's radius, against a
+// margin of a few hundred m^2).  Pairs inside the margin, and every pair
+// that passes, are decided with the exact expression, so the outcome is
+// that of the reference's comparisons bit for bit.
+WQ_HD_INLINE void lin_coeffs(const DPoint& p, double c[6]) {
+    const double r = p.r, a = p.th, b = p.ph;
+    const double rs = r * p.s;
+    c[0] = -2.0 * r + 4.0 * r * (a * a);
+    c[1] = -8.0 * r * a;
+    c[2] = 4.0 * r;
+    c[3] = 4.0 * rs * (b * b);
+    c[4] = -8.0 * rs * b;
+    c[5] = 4.0 * rs;
+}
+
+struct LinForm {
+    double k, a[6];
+};
+
+WQ_HD_INLINE LinForm lin_diff(const DPoint& q, const DPoint& p) {
+    double cq[6], cp[6];
+    lin_coeffs(q, cq);
+    lin_coeffs(p, cp);
+    LinForm L;
+    L.k = (q.r - p.r) * (q.r + p.r);
+    for (int j = 0; j < 6; ++j) L.a[j] = cq[j] - cp[j];
+    return L;
+}
+
+WQ_HD_INLINE double lin_eval(const LinForm& L, const double f[6]) {
+    double d = ::fma(L.a[0], f[0], L.k);
+    for (int j = 1; j < 6; ++j) d = ::fma(L.a[j], f[j], d);
+    return d;
+}
+
 struct DetectRay {
     DPoint cur, nxt, prv;
     DPoint nb[4];      // de-1, de+1, az-1, az+1 at the current time
     bool use[4];       // neighbour not skipped by is_closest_ray (non-branch targets)
     bool upper[4];     // nde == 2 || naz == 2: rejection on <= instead of <
+    LinForm next, prev;  // d2(next) - d2(cur), d2(prev) - d2(cur)
+    LinForm window;      // next, shifted to the centre of its pass window
+    double window_w;     // half width of that window
 };
+
+// The pre-test needs d2(next) - d2(cur) > 0 and d2(prev) - d2(cur) >= 0.
+// Their sum, the second difference of d2 along the ray, has small
+// coefficients, so an interval bound of it over the box of target features
+// (S_up) is tight; any target whose next-difference lies outside
+// [-margin, S_up + 2 margin] fails the pre-test.  One linear form and one
+// compare against that window reject almost every pair.
+WQ_HD_INLINE void detect_window(const FanConst& F, DetectRay& r) {
+    const double m = F.lin_margin;
+    double up = r.next.k + r.prev.k;
+    for (int j = 0; j < 6; ++j) {
+        const double a = r.next.a[j] + r.prev.a[j];
+        up += a >= 0.0 ? a * F.lin_fmax[j] : a * F.lin_fmin[j];
+    }
+    const double lo = -m, hi = up + 2.0 * m;
+    const double c = 0.5 * (lo + hi);
+    r.window = r.next;
+    r.window.k = r.next.k - c;
+    r.window_w = 0.5 * (hi - lo) + 1.0;
+    if (!(r.window_w < 1e300)) r.window_w = __builtin_inf();  // NaN / overflow: no rejection
+}
+
+// false when the window test rejects the pair (NaN passes through)
+WQ_HD_INLINE bool detect_window_ok(const DetectRay& r, const double f[6]) {
+    return !(fabs(lin_eval(r.window, f)) > r.window_w);
+}
 
 // Loads one interior, not-on-edge ray (the rays detect_eigenrays visits).
 WQ_HD_INLINE bool detect_ray(const FanConst& F, const Arrays& A, const Slots& S, int64_t i,
@@ -738,16 +818,33 @@ WQ_HD_INLINE bool detect_ray(const FanConst& F, const Arrays& A, const Slots& S,
         r.upper[n] = nd[n] == 2 || nz[n] == 2;
         r.nb[n] = dpoint(A, at(S.curr, N, j));
     }
+    r.next = lin_diff(r.nxt, r.cur);
+    r.prev = lin_diff(r.prv, r.cur);
+    detect_window(F, r);
     return true;
 }
 
-// Time-CPA pre-test (next > center, prev >= center; wave_queue.cc:384-394)
-// and, for targets without the source-branch rule, the current-time part of
+// First stage of a (target, ray) pair: false when the linear forms show
+// that the time-CPA pre-test fails (next <= center or prev < center beyond
+// the rounding margin); NaN estimates fall through to the exact test.
+WQ_HD_INLINE bool detect_maybe(const DetectRay& r, const double f[6], double margin, bool& time_sure) {
+    const double dn = lin_eval(r.next, f);
+    const double dp = lin_eval(r.prev, f);
+    if (dn < -margin || dp < -margin) return false;
+    time_sure = dn > margin && dp > margin;
+    return true;
+}
+
+// Exact second stage: the time-CPA pre-test (next > center, prev >= center;
+// wave_queue.cc:384-394) unless the linear forms already settled it, and,
+// for targets without the source-branch rule, the current-time part of
 // is_closest_ray for the four in-plane neighbours (wave_queue.cc:458-478).
-WQ_HD_INLINE bool detect_pair(const DetectRay& r, const DTarget& t, bool branch) {
+WQ_HD_INLINE bool detect_pair(const DetectRay& r, const DTarget& t, bool branch, bool time_sure) {
     const double center = dist2_fast(r.cur, t);
-    if (!(dist2_fast(r.nxt, t) > center)) return false;
-    if (dist2_fast(r.prv, t) < center) return false;
+    if (!time_sure) {
+        if (!(dist2_fast(r.nxt, t) > center)) return false;
+        if (dist2_fast(r.prv, t) < center) return false;
+    }
     if (branch) return true;
     for (int n = 0; n < 4; ++n) {
         if (!r.use[n]) continue;
@@ -755,6 +852,12 @@ WQ_HD_INLINE bool detect_pair(const DetectRay& r, const DTarget& t, bool branch)
         if (r.upper[n] ? (d <= center) : (d < center)) return false;
     }
     return true;
+}
+
+// The same decision without the linear forms (used by tests of the
+// pre-filter).
+WQ_HD_INLINE bool detect_pair_reference(const DetectRay& r, const DTarget& t, bool branch) {
+    return detect_pair(r, t, branch, false);
 }
 
 WQ_HD_INLINE DTarget dtarget(const FanConst& F, int64_t t) {
