@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 W_TGT = 17.0              # FP64 ops per (target, ray, step) of the distance test
-KERNELS = ["k_init", "k_reflect", "k_advance", "k_detect", "k_build"]
+KERNELS = ["k_init", "k_reflect", "k_advance", "k_detect", "k_build", "k_exact"]
 
 
 def log(msg):
@@ -96,7 +96,7 @@ def main():
     rays, _, _ = q.drain_eigenrays()
     n_rays = sc.num_rays()
     value = n_rays * args.steps / elapsed
-    kernel_ms = {KERNELS[k]: ms[k] for k in range(5) if launches[k]}
+    kernel_ms = {KERNELS[k]: ms[k] for k in range(len(KERNELS)) if launches[k]}
     # dominant kernel: the target search
     det_ms = ms[3]
     flops = W_TGT * sc.num_targets() * n_rays

@@ -281,6 +281,11 @@ wq3d_status wq3d_set_timing(wq3d_fan* fan, int32_t enable);
 wq3d_status wq3d_kernel_times(wq3d_fan* fan, double* ms_per_launch, int64_t* launches,
                               int32_t num_kernels);
 
+/* Diagnostics with no counterpart in the reference: the eigenray search's
+ * pair counts of the last step (pairs inside the pre-test window, pairs that
+ * passed the exact pre-test).  Synchronises the fan's stream. */
+wq3d_status wq3d_search_stats(wq3d_fan* fan, int64_t* window_hits, int64_t* candidates);
+
 #ifdef __cplusplus
 }
 #endif

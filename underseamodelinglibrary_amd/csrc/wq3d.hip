@@ -1,6 +1,6 @@
 // wq3d.hip - HIP kernels (gfx950, FP64) and the C ABI of libwq3d.so.
 //
-// One wave_queue::step() (waveq3d/wave_queue.cc:155-192) is four launches on
+// One wave_queue::step() (waveq3d/wave_queue.cc:155-192) is five launches on
 // the fan's stream:
 //   k_reflect  detect_reflections without find_edges (wave_queue.cc:197-292,
 //              reflection_model.cc:33-360): one thread per ray; the reference's
@@ -13,8 +13,12 @@
 //              (ode_integ.cc:131-186, wave_front.cc:125-178), attenuation and
 //              counter carry (wave_queue.cc:174-183), next step's flag[].
 //   k_detect   eigenray time-CPA pre-test (wave_queue.cc:372-395) for every
-//              (target, interior ray) pair, distances evaluated on the fly
-//              (wave_front.cc:222-236), survivors compacted per wavefront.
+//              (target, interior ray) pair, first as a window on one linear
+//              form of the target distances (wave_front.cc:222-236), which
+//              rejects almost every pair; the rest go to a hit list.
+//   k_exact    the exact pre-test of every hit, plus the current-time part
+//              of is_closest_ray for the in-plane neighbours; survivors go
+//              to the candidate list.
 //   k_build    is_closest_ray + build_eigenray (wave_queue.cc:411-702) with
 //              the hybrid Gaussian spreading walk per candidate.
 // Data layout: structure of arrays, slot-major ([slot][ray], ray = de*naz+az
@@ -60,17 +64,13 @@ __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A
     if (i < F.N) body_advance(i, E, F, A, S);
 }
 
-// Eigenray search, first stage (see detect_ray, detect_window_ok,
-// detect_maybe, detect_pair): one thread per interior ray; this block's
-// target group is staged through LDS in tiles (linear-form features and the
-// exact terms of the distance expression).  The main loop runs one linear
-// form and one compare per (ray, target) pair; the few pairs inside the
-// window go to a per-lane queue in LDS, drained once per tile (or when a
-// queue fills) so the exact second stage runs with the lanes of a
-// wavefront in step instead of on divergent branches.  Survivors are
-// compacted per wavefront into an LDS buffer and flushed to the global
-// candidate list with one atomic per block and tile.
-constexpr int kCandBuf = 1024;
+// Eigenray search, first stage: one thread per interior ray over this
+// block's target group, one linear form and one compare per (ray, target)
+// pair (detect_window_ok), with the target's features wave-uniform in SGPRs
+// (scalar loads).  The few pairs inside the window are queued per lane in
+// LDS and written to the global hit list a wavefront at a time (one atomic
+// per wavefront and flush).  Only the window form stays live, which keeps
+// the register count, and so the occupancy, where the scan needs it.
 constexpr int kQueue = 16;
 
 struct alignas(16) LinFeatures {
@@ -82,121 +82,92 @@ typedef __attribute__((address_space(4))) const LinFeatures ConstLinFeatures;  /
 typedef const LinFeatures ConstLinFeatures;
 #endif
 
-__device__ __forceinline__ void append_candidate(bool pass, unsigned long long key, int lane,
-                                                 unsigned int* s_count, unsigned long long* s_buf,
-                                                 const Records& Rc) {
-    const unsigned long long mask = __ballot(pass);
-    if (!mask) return;
-    const int leader = __ffsll((long long)mask) - 1;
-    unsigned int off = 0;
-    if (lane == leader) off = atomicAdd(s_count, (unsigned int)__popcll(mask));
-    off = __shfl(off, leader);
-    if (pass) {
-        const unsigned int pos = off + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
-        if (pos < (unsigned int)kCandBuf) {
-            s_buf[pos] = key;
-        } else {  // LDS buffer full: append directly
-            const unsigned long long g = atomicAdd(Rc.cand_count, 1ull);
-            if ((int64_t)g < Rc.cand_cap)
-                Rc.cand[g] = key;
-            else
-                atomicOr(Rc.error, 2);
-        }
-    }
-}
-
 __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S, Records Rc,
                                                   int64_t tgt_per_group) {
-    __shared__ LinFeatures s_f[kTargetTile + 1];
-    __shared__ DTarget s_x[kTargetTile];
-    __shared__ uint8_t s_br[kTargetTile];
-    __shared__ uint16_t s_q[kQueue][kBlock];
-    __shared__ unsigned long long s_buf[kCandBuf];
-    __shared__ unsigned int s_count;
-    __shared__ unsigned long long s_base;
+    __shared__ uint32_t s_q[kQueue][kBlock];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    DetectRay r;
-    const bool active = detect_ray(F, A, S, i, r);
-    const double margin = F.lin_margin;
+    DetectWindow W;
+    const bool active = detect_ray_window(F, A, S, i, W);
+    if (!__any(active)) return;  // wave-uniform: no barriers below
     const int64_t t_begin = (int64_t)blockIdx.y * tgt_per_group;
     const int64_t t_end = (F.nt < t_begin + tgt_per_group) ? F.nt : t_begin + tgt_per_group;
     const int lane = threadIdx.x & 63;
-    if (threadIdx.x == 0) s_count = 0;
     int qn = 0;
-    // exact second stage for this lane's queued targets of the tile at t0
-    auto drain = [&](int64_t t0) {
-        int maxq = qn;
-        for (int off = 32; off > 0; off >>= 1) maxq = max(maxq, __shfl_xor(maxq, off));
-        for (int j = 0; j < maxq; ++j) {
-            bool pass = false;
-            int k = 0;
-            if (j < qn) {
-                k = s_q[j][threadIdx.x];
-                bool time_sure = false;
-                if (detect_maybe(r, s_f[k].f, margin, time_sure))
-                    pass = detect_pair(r, s_x[k], s_br[k] != 0, time_sure);
-            }
-            append_candidate(pass, ((unsigned long long)(t0 + k) << 32) | (unsigned long long)i, lane,
-                             &s_count, s_buf, Rc);
+    // write this wavefront's queued hits to the global list
+    auto flush = [&]() {
+        int incl = qn;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        const int total = __shfl(incl, 63);
+        if (total == 0) return;
+        unsigned long long base = 0;
+        if (lane == 63) base = atomicAdd(Rc.hit_count, (unsigned long long)total);
+        base = __shfl(base, 63);
+        const unsigned long long mine = base + (unsigned long long)(incl - qn);
+        for (int j = 0; j < qn; ++j) {
+            const unsigned long long at_ = mine + (unsigned long long)j;
+            if ((int64_t)at_ < Rc.hit_cap)
+                Rc.hit[at_] = ((unsigned long long)s_q[j][threadIdx.x] << 32) | (unsigned long long)i;
+            else
+                atomicOr(Rc.error, 2);
         }
         qn = 0;
     };
-    for (int64_t t0 = t_begin; t0 < t_end; t0 += kTargetTile) {
-        const int tn = (int)((t_end - t0 < (int64_t)kTargetTile) ? t_end - t0 : (int64_t)kTargetTile);
-        __syncthreads();
-        {
-            double* flat = &s_f[0].f[0];
-            const double* src = F.tgt_lin + t0 * 6;
-            for (int k = threadIdx.x; k < tn * 6; k += blockDim.x) flat[k] = src[k];
-            for (int k = threadIdx.x; k < tn; k += blockDim.x) {
-                s_x[k] = dtarget(F, t0 + k);
-                s_br[k] = F.tgt_branch[t0 + k];
-            }
+    const ConstLinFeatures* cf = (const ConstLinFeatures*)(F.tgt_lin);
+    auto visit = [&](const LinFeatures& fc, int64_t t) {
+        if (active && detect_window_ok(W, fc.f)) {
+            s_q[qn][threadIdx.x] = (uint32_t)t;
+            ++qn;
         }
-        __syncthreads();
-        if (__any(active)) {
-            // the target of an iteration is the same for every lane: its
-            // features come through the scalar cache into SGPRs
-            const ConstLinFeatures* cf = (const ConstLinFeatures*)(F.tgt_lin) + t0;
-            auto visit = [&](const LinFeatures& fc, int k) {
-                if (active && detect_window_ok(r, fc.f)) {
-                    s_q[qn][threadIdx.x] = (uint16_t)k;
-                    ++qn;
-                }
-            };
-            int k = 0;
-            // groups of four targets: one batch of scalar loads, at most four
-            // queue entries (the queue is drained while it has room for them)
-            for (; k + 4 <= tn; k += 4) {
-                const LinFeatures f0 = cf[k], f1 = cf[k + 1], f2 = cf[k + 2], f3 = cf[k + 3];
-                visit(f0, k);
-                visit(f1, k + 1);
-                visit(f2, k + 2);
-                visit(f3, k + 3);
-                if (__any(qn > kQueue - 4)) drain(t0);
-            }
-            for (; k < tn; ++k) {
-                visit(cf[k], k);
-                if (__any(qn == kQueue)) drain(t0);
-            }
-            drain(t0);
+    };
+    // groups of four targets: one batch of scalar loads, at most four queue
+    // entries; the queues are flushed while they have room for a group
+    int64_t t = t_begin;
+    for (; t + 4 <= t_end; t += 4) {
+        const LinFeatures f0 = cf[t], f1 = cf[t + 1], f2 = cf[t + 2], f3 = cf[t + 3];
+        visit(f0, t);
+        visit(f1, t + 1);
+        visit(f2, t + 2);
+        visit(f3, t + 3);
+        if (__any(qn > kQueue - 4)) flush();
+    }
+    for (; t < t_end; ++t) {
+        visit(cf[t], t);
+        if (__any(qn == kQueue)) flush();
+    }
+    flush();
+}
+
+// Eigenray search, second stage: the exact pre-test (detect_pair) of every
+// hit, one thread per hit (grid-stride over the device-side count);
+// survivors are appended to the candidate list one atomic per wavefront.
+__global__ void __launch_bounds__(kBlock) k_exact(FanConst F, Arrays A, Slots S, Records Rc) {
+    const unsigned long long raw = *Rc.hit_count;
+    const int64_t nhit = (int64_t)(raw < (unsigned long long)Rc.hit_cap ? raw : (unsigned long long)Rc.hit_cap);
+    const ArrayPoints pts{A, S, F.N};
+    const int lane = threadIdx.x & 63;
+    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < nhit;
+         h += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long key = Rc.hit[h];
+        const int64_t t = (int64_t)(key >> 32);
+        const int64_t i = (int64_t)(key & 0xffffffffull);
+        DetectRay r;
+        const bool pass = detect_ray(F, A, S, i, r) && detect_pair(r, pts, dtarget(F, t), F.tgt_branch[t] != 0);
+        const unsigned long long mask = __ballot(pass);
+        if (!mask) continue;
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(Rc.cand_count, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader);
+        if (pass) {
+            const unsigned long long at_ = base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+            if ((int64_t)at_ < Rc.cand_cap)
+                Rc.cand[at_] = key;
+            else
+                atomicOr(Rc.error, 2);
         }
-        // flush this tile's candidates
-        __syncthreads();
-        const unsigned int n = s_count < (unsigned int)kCandBuf ? s_count : (unsigned int)kCandBuf;
-        if (n > 0) {
-            if (threadIdx.x == 0) s_base = atomicAdd(Rc.cand_count, (unsigned long long)n);
-            __syncthreads();
-            const unsigned long long base = s_base;
-            for (unsigned int k = threadIdx.x; k < n; k += blockDim.x) {
-                if ((int64_t)(base + k) < Rc.cand_cap)
-                    Rc.cand[base + k] = s_buf[k];
-                else
-                    atomicOr(Rc.error, 2);
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) s_count = 0;
     }
 }
 
@@ -329,20 +300,42 @@ void launch_detect(hipStream_t st, unsigned grid, unsigned groups, const FanCons
     hipLaunchKernelGGL(k_detect, dim3(grid, groups), dim3(kBlock), 0, st, F, A, S, R, per_group);
 #else
     for (int64_t i = 0; i < F.N; ++i) {
-        DetectRay r;
-        if (!detect_ray(F, A, S, i, r)) continue;
+        DetectWindow W;
+        if (!detect_ray_window(F, A, S, i, W)) continue;
         for (int64_t t = 0; t < F.nt; ++t) {
-            bool time_sure = false;
-            if (!detect_window_ok(r, F.tgt_lin + t * 6)) continue;
-            if (!detect_maybe(r, F.tgt_lin + t * 6, F.lin_margin, time_sure)) continue;
-            if (!detect_pair(r, dtarget(F, t), F.tgt_branch[t] != 0, time_sure)) continue;
-            const unsigned long long off = (*R.cand_count)++;
-            if ((int64_t)off < R.cand_cap)
-                R.cand[off] = ((unsigned long long)t << 32) | (unsigned long long)i;
+            if (!detect_window_ok(W, F.tgt_lin + t * 6)) continue;
+            const unsigned long long off = (*R.hit_count)++;
+            if ((int64_t)off < R.hit_cap)
+                R.hit[off] = ((unsigned long long)t << 32) | (unsigned long long)i;
             else
                 *R.error |= 2;
         }
     }
+#endif
+}
+
+void launch_exact(hipStream_t st, const FanConst& F, const Arrays& A, const Slots& S, const Records& R) {
+#ifndef WQ3D_EMULATE
+    hipLaunchKernelGGL(k_exact, dim3(2048), dim3(kBlock), 0, st, F, A, S, R);
+#else
+    const ArrayPoints pts{A, S, F.N};
+    const int64_t n = (int64_t)std::min<unsigned long long>(*R.hit_count, (unsigned long long)R.hit_cap);
+    if (std::getenv("WQ3D_EMU_TRACE"))
+        std::fprintf(stderr, "[emu] hits %lld (cap %lld), candidates before %llu\n", (long long)*R.hit_count,
+                     (long long)R.hit_cap, *R.cand_count);
+    for (int64_t h = 0; h < n; ++h) {
+        const unsigned long long key = R.hit[h];
+        const int64_t t = (int64_t)(key >> 32);
+        const int64_t i = (int64_t)(key & 0xffffffffull);
+        DetectRay r;
+        if (!detect_ray(F, A, S, i, r) || !detect_pair(r, pts, dtarget(F, t), F.tgt_branch[t] != 0)) continue;
+        const unsigned long long off = (*R.cand_count)++;
+        if ((int64_t)off < R.cand_cap)
+            R.cand[off] = key;
+        else
+            *R.error |= 2;
+    }
+    if (std::getenv("WQ3D_EMU_TRACE")) std::fprintf(stderr, "[emu] candidates %llu\n", *R.cand_count);
 #endif
 }
 
@@ -464,7 +457,7 @@ struct wq3d_fan {
     std::vector<wq3d_eigenverb> h_verbs;
     std::vector<double> h_verb_power;
     std::vector<wq3d_reflection> h_refl;
-    unsigned long long* counters = nullptr;  // device: ray, verb, refl, cand
+    unsigned long long* counters = nullptr;  // device: ray, verb, refl, cand, hit
     int32_t* error = nullptr;
     int64_t pending_steps = 0;  // steps since the last device->host record move
     // timing
@@ -585,7 +578,7 @@ hipEvent_t take_event(wq3d_fan* f) {
 }
 
 // kernel ids for timing
-enum { KID_INIT = 0, KID_REFLECT = 1, KID_ADVANCE = 2, KID_DETECT = 3, KID_BUILD = 4, KID_COUNT = 5 };
+enum { KID_INIT = 0, KID_REFLECT = 1, KID_ADVANCE = 2, KID_DETECT = 3, KID_BUILD = 4, KID_EXACT = 5, KID_COUNT = 6 };
 
 template <class Launch>
 hipError_t timed(wq3d_fan* f, int kid, Launch&& launch) {
@@ -930,22 +923,24 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         R.ray_cap = std::max<int64_t>(1 << 16, std::min<int64_t>(nt * 64, 1 << 22));
         R.verb_cap = std::max<int64_t>(1 << 16, std::min<int64_t>(N * 4, 1 << 22));
         R.refl_cap = std::max<int64_t>(1 << 16, std::min<int64_t>(N * 4, 1 << 22));
-        R.cand_cap = nt > 0 ? std::max<int64_t>(1 << 20, std::min<int64_t>(nt * N / 16, 1 << 26)) : 1;
+        R.hit_cap = nt > 0 ? std::max<int64_t>(1 << 20, std::min<int64_t>(nt * N / 8, 1 << 27)) : 1;
+        R.cand_cap = nt > 0 ? std::max<int64_t>(1 << 20, std::min<int64_t>(nt * N / 64, 1 << 24)) : 1;
         if (f->mem.alloc(&R.rays, R.ray_cap) != hipSuccess ||
             f->mem.alloc(&R.ray_int, R.ray_cap * F.nf) != hipSuccess ||
             f->mem.alloc(&R.ray_phase, R.ray_cap * F.nf) != hipSuccess ||
             f->mem.alloc(&R.verbs, R.verb_cap) != hipSuccess ||
             f->mem.alloc(&R.verb_power, R.verb_cap * F.nf) != hipSuccess ||
             f->mem.alloc(&R.refl, R.refl_cap) != hipSuccess ||
-            f->mem.alloc(&R.cand, R.cand_cap) != hipSuccess ||
-            f->mem.alloc(&f->counters, 4) != hipSuccess || f->mem.alloc(&f->error, 1) != hipSuccess)
+            f->mem.alloc(&R.cand, R.cand_cap) != hipSuccess || f->mem.alloc(&R.hit, R.hit_cap) != hipSuccess ||
+            f->mem.alloc(&f->counters, 5) != hipSuccess || f->mem.alloc(&f->error, 1) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "record allocation failed"));
         R.ray_count = f->counters + 0;
         R.verb_count = f->counters + 1;
         R.refl_count = f->counters + 2;
         R.cand_count = f->counters + 3;
+        R.hit_count = f->counters + 4;
         R.error = f->error;
-        if (rt::memset(f->counters, 0, 4 * sizeof(unsigned long long)) != hipSuccess ||
+        if (rt::memset(f->counters, 0, 5 * sizeof(unsigned long long)) != hipSuccess ||
             rt::memset(f->error, 0, sizeof(int32_t)) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "memset failed"));
     }
@@ -1017,10 +1012,12 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
         }
         if (F.nt > 0 && F.has_spreading) {
             const Slots S = slots_of(f);
-            HIP_TRY(rt::memset_async(f->R.cand_count, 0, sizeof(unsigned long long), f->stream));
+            // candidate and hit counters are adjacent
+            HIP_TRY(rt::memset_async(f->R.cand_count, 0, 2 * sizeof(unsigned long long), f->stream));
             HIP_TRY(timed(f, KID_DETECT, [&] {
                 launch_detect(f->stream, grid, (unsigned)groups, F, f->A, S, f->R, per_group);
             }));
+            HIP_TRY(timed(f, KID_EXACT, [&] { launch_exact(f->stream, F, f->A, S, f->R); }));
             const double t = f->time;
             const int64_t key = f->steps;
             HIP_TRY(timed(f, KID_BUILD, [&] {
@@ -1202,6 +1199,16 @@ wq3d_status wq3d_snapshot(wq3d_fan* f, int32_t slot, wq3d_front_view* v) {
     return WQ3D_OK;
 }
 
+wq3d_status wq3d_search_stats(wq3d_fan* f, int64_t* window_hits, int64_t* candidates) {
+    if (!f || !window_hits || !candidates) return fail(WQ3D_ERR_ARGUMENT, "null argument");
+    HIP_TRY(rt::sync(f->stream));
+    unsigned long long c[2] = {0, 0};
+    if (f->R.cand_count) HIP_TRY(rt::memcpy(c, f->R.cand_count, sizeof(c), hipMemcpyDeviceToHost));
+    *candidates = (int64_t)c[0];
+    *window_hits = (int64_t)c[1];
+    return WQ3D_OK;
+}
+
 wq3d_status wq3d_set_timing(wq3d_fan* f, int32_t enable) {
     if (!f) return fail(WQ3D_ERR_ARGUMENT, "null fan");
     f->timing = enable != 0;
@@ -1234,7 +1241,7 @@ wq3d_status wq3d_kernel_times(wq3d_fan* f, double* ms, int64_t* launches, int32_
 // the earth's radius, plus targets placed by bisection on the boundaries
 // where an exact comparison of the pre-test flips, decided once with the
 // linear-form pre-filter and once with the exact expressions alone.
-// counts = {pairs, mismatches, pairs settled by the exact fallback, passes}.
+// counts = {pairs, mismatches, pairs inside the window, passes}.
 static double emu_uniform(unsigned long long& s, double lo, double hi) {
     s = s * 6364136223846793005ull + 1442695040888963407ull;
     return lo + (hi - lo) * ((double)(s >> 11) * 0x1.0p-53);
@@ -1269,20 +1276,28 @@ int32_t wq3d_emu_check_detect(int64_t num_rays, int64_t num_targets, uint64_t se
             return emu_point(r + k * vu + bend * k * k, t - k * vn / r, p + k * ve / (r * std::sin(t)));
         };
         const double bend = emu_uniform(s, -2.0, 2.0);
+        struct FuzzPoints {
+            DPoint n, p, nbp[4];
+            DPoint nxt(int64_t) const { return n; }
+            DPoint prv(int64_t) const { return p; }
+            DPoint nb(int64_t j) const { return nbp[j]; }
+        } fp;
         DetectRay dr;
+        dr.i = 0;
         dr.cur = at_step(0.0, bend);
-        dr.nxt = at_step(1.0, bend);
-        dr.prv = at_step(-1.0, bend);
+        fp.n = at_step(1.0, bend);
+        fp.p = at_step(-1.0, bend);
         const int nz[4] = {1, 1, 0, 2};
         const int nd[4] = {0, 2, 1, 1};
+        dr.use = 0;
+        dr.upper = 0;
         for (int n = 0; n < 4; ++n) {
-            dr.nb[n] = emu_point(r + emu_uniform(s, -300.0, 300.0), t + emu_uniform(s, -300.0, 300.0) / r,
-                                 p + emu_uniform(s, -300.0, 300.0) / (r * std::sin(t)));
-            dr.use[n] = emu_uniform(s, 0.0, 1.0) < 0.8;
-            dr.upper[n] = nd[n] == 2 || nz[n] == 2;
+            fp.nbp[n] = emu_point(r + emu_uniform(s, -300.0, 300.0), t + emu_uniform(s, -300.0, 300.0) / r,
+                                  p + emu_uniform(s, -300.0, 300.0) / (r * std::sin(t)));
+            dr.nbj[n] = n;
+            dr.use |= (emu_uniform(s, 0.0, 1.0) < 0.8 ? 1u : 0u) << n;
+            dr.upper |= ((nd[n] == 2 || nz[n] == 2) ? 1u : 0u) << n;
         }
-        dr.next = lin_diff(dr.nxt, dr.cur);
-        dr.prev = lin_diff(dr.prv, dr.cur);
         auto make = [&](double tr, double tt, double tp, DTarget& x, double f6[6]) {
             x.th = 0.5 * tt;
             x.ph = 0.5 * tp;
@@ -1298,7 +1313,7 @@ int32_t wq3d_emu_check_detect(int64_t num_rays, int64_t num_targets, uint64_t se
             double f6[6];
             make(tr, tt, tp, x, f6);
             const double c = dist2_fast(dr.cur, x);
-            const DPoint& q = which == 0 ? dr.nxt : which == 1 ? dr.prv : dr.nb[which - 2];
+            const DPoint q = which == 0 ? fp.n : which == 1 ? fp.p : fp.nbp[which - 2];
             return dist2_fast(q, x) > c;
         };
         struct Tgt {
@@ -1346,18 +1361,17 @@ int32_t wq3d_emu_check_detect(int64_t num_rays, int64_t num_targets, uint64_t se
                 F.lin_fmax[j] = std::max(F.lin_fmax[j], f6[j]);
             }
         }
-        detect_window(F, dr);
+        const DetectWindow W = detect_window(F, lin_diff(fp.n, dr.cur), lin_diff(fp.p, dr.cur));
         for (const Tgt& g : tg) {
             DTarget x;
             double f6[6];
             make(g.r, g.t, g.p, x, f6);
-            const bool ref = detect_pair_reference(dr, x, g.branch);
-            bool sure = false, got = false, maybe = false;
-            if (detect_window_ok(dr, f6)) maybe = detect_maybe(dr, f6, margin, sure);
-            if (maybe) got = detect_pair(dr, x, g.branch, sure);
+            const bool ref = detect_pair(dr, fp, x, g.branch);
+            const bool window = detect_window_ok(W, f6);
+            const bool got = window && detect_pair(dr, fp, x, g.branch);
             counts[0] += 1;
             counts[1] += ref != got;
-            counts[2] += maybe && !sure;
+            counts[2] += window;
             counts[3] += ref;
         }
     }

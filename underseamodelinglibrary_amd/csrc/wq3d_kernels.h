@@ -104,7 +104,10 @@ struct Records {
     wq3d_reflection* refl;
     unsigned long long* refl_count;
     int64_t refl_cap;
-    unsigned long long* cand;
+    unsigned long long* hit;    // (target, ray) pairs inside the pre-test window
+    unsigned long long* hit_count;
+    int64_t hit_cap;
+    unsigned long long* cand;   // pairs that passed the exact pre-test
     unsigned long long* cand_count;
     int64_t cand_cap;
     int32_t* error;             // bit 0: record overflow, bit 1: candidate overflow,
@@ -750,13 +753,20 @@ WQ_HD_INLINE double lin_eval(const LinForm& L, const double f[6]) {
 }
 
 struct DetectRay {
-    DPoint cur, nxt, prv;
-    DPoint nb[4];      // de-1, de+1, az-1, az+1 at the current time
-    bool use[4];       // neighbour not skipped by is_closest_ray (non-branch targets)
-    bool upper[4];     // nde == 2 || naz == 2: rejection on <= instead of <
-    LinForm next, prev;  // d2(next) - d2(cur), d2(prev) - d2(cur)
-    LinForm window;      // next, shifted to the centre of its pass window
-    double window_w;     // half width of that window
+    int64_t i;           // ray index
+    DPoint cur;
+    int32_t nbj[4];      // neighbours de-1, de+1, az-1, az+1 (AZ wrap of is_closest_ray)
+    uint32_t use;        // bit n: neighbour n not skipped by is_closest_ray (non-branch targets)
+    uint32_t upper;      // bit n: nde == 2 || naz == 2, rejection on <= instead of <
+};
+
+// The pre-test window of one ray: d2(next) - d2(cur) shifted to the centre
+// of the interval it must lie in, and the half width of that interval.
+struct DetectWindow {
+    LinForm form;
+    double w;
+    LinForm prev;  // d2(prev) - d2(cur)
+    double m;      // rounding margin
 };
 
 // The pre-test needs d2(next) - d2(cur) > 0 and d2(prev) - d2(cur) >= 0.
@@ -764,44 +774,82 @@ struct DetectRay {
 // coefficients, so an interval bound of it over the box of target features
 // (S_up) is tight; any target whose next-difference lies outside
 // [-margin, S_up + 2 margin] fails the pre-test.  One linear form and one
-// compare against that window reject almost every pair.
-WQ_HD_INLINE void detect_window(const FanConst& F, DetectRay& r) {
+// compare against that window reject almost every pair; the prev form then
+// settles most of the rest.
+WQ_HD_INLINE DetectWindow detect_window(const FanConst& F, const LinForm& next, const LinForm& prev) {
     const double m = F.lin_margin;
-    double up = r.next.k + r.prev.k;
+    double up = next.k + prev.k;
     for (int j = 0; j < 6; ++j) {
-        const double a = r.next.a[j] + r.prev.a[j];
+        const double a = next.a[j] + prev.a[j];
         up += a >= 0.0 ? a * F.lin_fmax[j] : a * F.lin_fmin[j];
     }
     const double lo = -m, hi = up + 2.0 * m;
     const double c = 0.5 * (lo + hi);
-    r.window = r.next;
-    r.window.k = r.next.k - c;
-    r.window_w = 0.5 * (hi - lo) + 1.0;
-    if (!(r.window_w < 1e300)) r.window_w = __builtin_inf();  // NaN / overflow: no rejection
+    DetectWindow W;
+    W.form = next;
+    W.form.k = next.k - c;
+    W.w = 0.5 * (hi - lo) + 1.0;
+    if (!(W.w < 1e300)) W.w = __builtin_inf();  // NaN / overflow: no rejection
+    W.prev = prev;
+    W.m = m;
+    return W;
 }
 
-// false when the window test rejects the pair (NaN passes through)
-WQ_HD_INLINE bool detect_window_ok(const DetectRay& r, const double f[6]) {
-    return !(fabs(lin_eval(r.window, f)) > r.window_w);
+// false when the window rejects the pair, or, for the pairs inside it, when
+// d2(prev) - d2(cur) is below -margin (prev surely closer than center).
+// The second form matters for rays whose window is wide, such as rays whose
+// three snapshots straddle a reflection.  NaN passes through.
+WQ_HD_INLINE bool detect_window_ok(const DetectWindow& W, const double f[6]) {
+    if (fabs(lin_eval(W.form, f)) > W.w) return false;
+    return !(lin_eval(W.prev, f) < -W.m);
 }
 
-// Loads one interior, not-on-edge ray (the rays detect_eigenrays visits).
-WQ_HD_INLINE bool detect_ray(const FanConst& F, const Arrays& A, const Slots& S, int64_t i,
-                             DetectRay& r) {
-    const int64_t N = F.N;
-    if (i >= N) return false;
+// Ray points of the exact stage, read from the state arrays on demand.
+struct ArrayPoints {
+    const Arrays& A;
+    const Slots& S;
+    int64_t N;
+    WQ_HD_INLINE DPoint nxt(int64_t i) const { return dpoint(A, at(S.next, N, i)); }
+    WQ_HD_INLINE DPoint prv(int64_t i) const { return dpoint(A, at(S.prev, N, i)); }
+    WQ_HD_INLINE DPoint nb(int64_t j) const { return dpoint(A, at(S.curr, N, j)); }
+};
+
+// The rays detect_eigenrays visits: interior and not on an edge.
+WQ_HD_INLINE bool detect_interior(const FanConst& F, const Arrays& A, const Slots& S, int64_t i) {
+    if (i < 0 || i >= F.N) return false;
     const int de = (int)(i / F.naz), az = (int)(i % F.naz);
     const int az_start = F.az_boundary ? 0 : 1;
+    if (!(de >= 1 && de < F.nde - 1 && az >= az_start && az < F.naz - 1)) return false;
+    return A.edge[at(S.curr, F.N, i)] == 0;
+}
+
+// Window of one ray for the first stage.
+WQ_HD_INLINE bool detect_ray_window(const FanConst& F, const Arrays& A, const Slots& S, int64_t i,
+                                    DetectWindow& W) {
+    if (!detect_interior(F, A, S, i)) return false;
+    const int64_t N = F.N;
+    const DPoint cur = dpoint(A, at(S.curr, N, i));
+    const DPoint nxt = dpoint(A, at(S.next, N, i));
+    const DPoint prv = dpoint(A, at(S.prev, N, i));
+    W = detect_window(F, lin_diff(nxt, cur), lin_diff(prv, cur));
+    return true;
+}
+
+// Ray data of the exact stage: current point and the in-plane neighbours.
+WQ_HD_INLINE bool detect_ray(const FanConst& F, const Arrays& A, const Slots& S, int64_t i,
+                             DetectRay& r) {
+    if (!detect_interior(F, A, S, i)) return false;
+    const int64_t N = F.N;
+    const int de = (int)(i / F.naz), az = (int)(i % F.naz);
     const int max_az = F.naz - 1;
-    if (!(de >= 1 && de < F.nde - 1 && az >= az_start && az < max_az)) return false;
-    if (A.edge[at(S.curr, N, i)] != 0) return false;
+    r.i = i;
     r.cur = dpoint(A, at(S.curr, N, i));
-    r.nxt = dpoint(A, at(S.next, N, i));
-    r.prv = dpoint(A, at(S.prev, N, i));
     // neighbour indices with the AZ wrap of is_closest_ray (wave_queue.cc:423-431)
     const int dd[4] = {de - 1, de + 1, de, de};
     const int nz[4] = {1, 1, 0, 2};
     const int nd[4] = {0, 2, 1, 1};
+    r.use = 0;
+    r.upper = 0;
     for (int n = 0; n < 4; ++n) {
         int a = az + nz[n] - 1;
         if (F.az_boundary) {
@@ -814,50 +862,30 @@ WQ_HD_INLINE bool detect_ray(const FanConst& F, const Arrays& A, const Slots& S,
         const int64_t j = (int64_t)dd[n] * F.naz + a;
         bool use = !((a == 0 && !F.az_boundary) || a == max_az);
         if (use && nd[n] != 1) use = A.edge[at(S.curr, N, j)] == 0;
-        r.use[n] = use;
-        r.upper[n] = nd[n] == 2 || nz[n] == 2;
-        r.nb[n] = dpoint(A, at(S.curr, N, j));
+        r.nbj[n] = (int32_t)j;
+        r.use |= (use ? 1u : 0u) << n;
+        r.upper |= ((nd[n] == 2 || nz[n] == 2) ? 1u : 0u) << n;
     }
-    r.next = lin_diff(r.nxt, r.cur);
-    r.prev = lin_diff(r.prv, r.cur);
-    detect_window(F, r);
     return true;
 }
 
-// First stage of a (target, ray) pair: false when the linear forms show
-// that the time-CPA pre-test fails (next <= center or prev < center beyond
-// the rounding margin); NaN estimates fall through to the exact test.
-WQ_HD_INLINE bool detect_maybe(const DetectRay& r, const double f[6], double margin, bool& time_sure) {
-    const double dn = lin_eval(r.next, f);
-    const double dp = lin_eval(r.prev, f);
-    if (dn < -margin || dp < -margin) return false;
-    time_sure = dn > margin && dp > margin;
-    return true;
-}
-
-// Exact second stage: the time-CPA pre-test (next > center, prev >= center;
-// wave_queue.cc:384-394) unless the linear forms already settled it, and,
-// for targets without the source-branch rule, the current-time part of
-// is_closest_ray for the four in-plane neighbours (wave_queue.cc:458-478).
-WQ_HD_INLINE bool detect_pair(const DetectRay& r, const DTarget& t, bool branch, bool time_sure) {
+// Exact test of one (target, ray) pair: the time-CPA pre-test (next >
+// center, prev >= center; wave_queue.cc:384-394) and, for targets without
+// the source-branch rule, the current-time part of is_closest_ray for the
+// four in-plane neighbours (wave_queue.cc:458-478).
+template <class Points>
+WQ_HD_INLINE bool detect_pair(const DetectRay& r, const Points& pts, const DTarget& t, bool branch) {
     const double center = dist2_fast(r.cur, t);
-    if (!time_sure) {
-        if (!(dist2_fast(r.nxt, t) > center)) return false;
-        if (dist2_fast(r.prv, t) < center) return false;
-    }
+    if (!(dist2_fast(pts.nxt(r.i), t) > center)) return false;
+    if (dist2_fast(pts.prv(r.i), t) < center) return false;
     if (branch) return true;
+#pragma unroll 1
     for (int n = 0; n < 4; ++n) {
-        if (!r.use[n]) continue;
-        const double d = dist2_fast(r.nb[n], t);
-        if (r.upper[n] ? (d <= center) : (d < center)) return false;
+        if (!((r.use >> n) & 1u)) continue;
+        const double d = dist2_fast(pts.nb(r.nbj[n]), t);
+        if (((r.upper >> n) & 1u) ? (d <= center) : (d < center)) return false;
     }
     return true;
-}
-
-// The same decision without the linear forms (used by tests of the
-// pre-filter).
-WQ_HD_INLINE bool detect_pair_reference(const DetectRay& r, const DTarget& t, bool branch) {
-    return detect_pair(r, t, branch, false);
 }
 
 WQ_HD_INLINE DTarget dtarget(const FanConst& F, int64_t t) {
