@@ -172,14 +172,17 @@ __global__ void __launch_bounds__(kBlock) k_exact(FanConst F, Arrays A, Slots S,
 }
 
 // is_closest_ray + build_eigenray per candidate, grid-stride over the
-// device-side candidate count
+// device-side candidate count; each candidate is built by a group of G
+// consecutive lanes, one per frequency (body_build<true>)
 __global__ void __launch_bounds__(kBlock) k_build(DEnv E, FanConst F, Arrays A, Slots S, Records Rc,
-                                                 double time, int64_t step_key) {
+                                                 double time, int64_t step_key, int group) {
     const unsigned long long raw = *Rc.cand_count;
     const int64_t ncand = (int64_t)(raw < (unsigned long long)Rc.cand_cap ? raw : (unsigned long long)Rc.cand_cap);
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncand;
-         c += (int64_t)gridDim.x * blockDim.x)
-        body_build(c, E, F, A, S, Rc, time, step_key);
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane_in_group = (int)(gid % group);
+    if (lane_in_group >= F.nf) return;  // lanes beyond the fan's frequencies idle
+    for (int64_t c = gid / group; c < ncand; c += (int64_t)gridDim.x * blockDim.x / group)
+        body_build<true>(c, lane_in_group, group, E, F, A, S, Rc, time, step_key);
 }
 #endif  // WQ3D_EMULATE
 
@@ -342,10 +345,12 @@ void launch_exact(hipStream_t st, const FanConst& F, const Arrays& A, const Slot
 void launch_build(hipStream_t st, const DEnv& E, const FanConst& F, const Arrays& A, const Slots& S,
                   const Records& R, double t, int64_t key) {
 #ifndef WQ3D_EMULATE
-    hipLaunchKernelGGL(k_build, dim3(1024), dim3(kBlock), 0, st, E, F, A, S, R, t, key);
+    int group = 1;
+    while (group < F.nf) group <<= 1;
+    hipLaunchKernelGGL(k_build, dim3(1024), dim3(kBlock), 0, st, E, F, A, S, R, t, key, group);
 #else
     const int64_t n = (int64_t)std::min<unsigned long long>(*R.cand_count, (unsigned long long)R.cand_cap);
-    for (int64_t c = 0; c < n; ++c) body_build(c, E, F, A, S, R, t, key);
+    for (int64_t c = 0; c < n; ++c) body_build<false>(c, -1, 1, E, F, A, S, R, t, key);
 #endif
 }
 

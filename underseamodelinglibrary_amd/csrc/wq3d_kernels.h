@@ -1141,12 +1141,26 @@ WQ_HD void compute_offsets(const BuildCtx& b, int64_t t, int de, int az, const d
 }
 
 // spreading_hybrid_gaussian (spreading_hybrid_gaussian.cc:20-571)
+// Frequencies one thread evaluates in body_build.  Split: the candidate is
+// built by a group of lanes, lane g evaluating frequency 0 (which steers the
+// spreading walks) and frequency g, in two register slots.  Not split: one
+// thread evaluates every frequency.
+template <bool Split>
+struct FreqSel {
+    static constexpr int kSlots = Split ? 2 : kMaxFreq;
+    int nf;
+    int fsel;  // Split only: this lane's frequency (0 .. nf-1)
+    WQ_HD_INLINE int count() const { return Split ? (fsel > 0 ? 2 : 1) : nf; }
+    WQ_HD_INLINE int freq(int k) const { return Split ? (k == 0 ? 0 : fsel) : k; }
+};
+
+template <bool Split>
 struct Spread {
     const BuildCtx& b;
-    int nf;
-    double spread[kMaxFreq];
-    double I_de[kMaxFreq];
-    double I_az[kMaxFreq];
+    FreqSel<Split> fs;
+    double spread[FreqSel<Split>::kSlots];
+    double I_de[FreqSel<Split>::kSlots];
+    double I_az[FreqSel<Split>::kSlots];
     uint32_t dup[kMaxAz / 32];
 };
 
@@ -1214,12 +1228,14 @@ WQ_HD double width_az(const BuildCtx& b, int de, int az, const double offset[3])
 }
 
 // spreading_hybrid_gaussian::gaussian (.h:148-153), accumulated into out
-WQ_HD_INLINE void gaussian_add(const Spread& s, double d, double w, double A,
-                                             double* out, bool assign) {
-    for (int f = 0; f < s.nf; ++f) {
-        const double bw = s.spread[f] + 2.0 * 2.0 * w * w;
+// (slot k holds frequency s.fs.freq(k))
+template <bool Split>
+WQ_HD_INLINE void gaussian_add(const Spread<Split>& s, double d, double w, double A, double* out, bool assign) {
+    for (int k = 0; k < FreqSel<Split>::kSlots; ++k) {
+        if (k >= s.fs.count()) break;
+        const double bw = s.spread[k] + 2.0 * 2.0 * w * w;
         const double g = wqm::exp((-0.5 * d * d) / bw) / sqrt(bw) * A;
-        out[f] = assign ? g : out[f] + g;
+        out[k] = assign ? g : out[k] + g;
     }
 }
 
@@ -1228,8 +1244,8 @@ WQ_HD_INLINE bool edge_at(const BuildCtx& b, int de, int az) {
 }
 
 // intensity_de (spreading_hybrid_gaussian.cc:124-258)
-WQ_HD void intensity_de(Spread& s, int de, int az, const double offset[3],
-                             const double distance[3]) {
+template <bool Split>
+WQ_HD void intensity_de(Spread<Split>& s, int de, int az, const double offset[3], const double distance[3]) {
     const BuildCtx& b = s.b;
     const double THRESHOLD = 1.002305238;
     const double* norm_de = b.F.norm_de;
@@ -1283,7 +1299,8 @@ WQ_HD void intensity_de(Spread& s, int de, int az, const double offset[3],
     }
 }
 
-WQ_HD_INLINE bool dup_test_set(Spread& s, int a) {
+template <bool Split>
+WQ_HD_INLINE bool dup_test_set(Spread<Split>& s, int a) {
     const uint32_t bit = 1u << (a & 31);
     const bool was = (s.dup[a >> 5] & bit) != 0;
     s.dup[a >> 5] |= bit;
@@ -1291,8 +1308,8 @@ WQ_HD_INLINE bool dup_test_set(Spread& s, int a) {
 }
 
 // intensity_az (spreading_hybrid_gaussian.cc:264-425)
-WQ_HD void intensity_az(Spread& s, int de, int az, const double offset[3],
-                             const double distance[3]) {
+template <bool Split>
+WQ_HD void intensity_az(Spread<Split>& s, int de, int az, const double offset[3], const double distance[3]) {
     const BuildCtx& b = s.b;
     const FanConst& F = b.F;
     const double THRESHOLD = 1.002305238;
@@ -1367,9 +1384,36 @@ WQ_HD_INLINE void ndir_angles(const Arrays& A, int64_t k, double& de, double& az
     az = to_degrees(wqm::atan2(np, -nt));
 }
 
-WQ_HD void body_build(int64_t c, const DEnv& E, const FanConst& F, const Arrays& A, const Slots& S,
-                     const Records& Rc, double time, int64_t step_key) {
+// Lane-group helpers of the split build: a candidate's lanes are
+// consecutive lanes of one wavefront, starting at a multiple of G.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ bool group_any(bool p, int G) {
+    const unsigned long long m = __ballot(p);
+    const int base = (int)(threadIdx.x & 63) & ~(G - 1);
+    const unsigned long long sel = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << base);
+    return (m & sel) != 0ull;
+}
+__device__ __forceinline__ unsigned long long group_bcast(unsigned long long v, int G) {
+    const int base = (int)(threadIdx.x & 63) & ~(G - 1);
+    const unsigned int lo = (unsigned int)__shfl((int)(unsigned int)(v & 0xffffffffull), base);
+    const unsigned int hi = (unsigned int)__shfl((int)(unsigned int)(v >> 32), base);
+    return ((unsigned long long)hi << 32) | lo;
+}
+#else
+inline bool group_any(bool p, int) { return p; }
+inline unsigned long long group_bcast(unsigned long long v, int) { return v; }
+#endif
+
+// is_closest_ray + build_eigenray (wave_queue.cc:411-702) for candidate c.
+// Split: called by the G lanes of a group with fsel = the lane's frequency;
+// every lane runs the same control flow (the walks stop on frequency 0,
+// which every lane evaluates), so the group stays converged.
+template <bool Split>
+WQ_HD void body_build(int64_t c, int fsel, int G, const DEnv& E, const FanConst& F, const Arrays& A,
+                      const Slots& S, const Records& Rc, double time, int64_t step_key) {
     const BuildCtx b{E, F, A, S};
+    const FreqSel<Split> fs{F.nf, fsel};
+    constexpr int kSlots = FreqSel<Split>::kSlots;
     {
         const unsigned long long key = Rc.cand[c];
         const int64_t t = (int64_t)(key >> 32);
@@ -1387,11 +1431,14 @@ WQ_HD void body_build(int64_t c, const DEnv& E, const FanConst& F, const Arrays&
         const double delta[3] = {F.dt, seq_inc(F.de, de), seq_inc(F.az, az)};
         double offset[3], distance[3];
         compute_offsets(b, t, de, az, D, delta, offset, distance);
-        Spread sp{b, F.nf};
+        Spread<Split> sp{b, fs};
         double ts_c, ts_g[3];
         sound_speed(E.profile, F.tgt_r[t], F.tgt_t[t], F.tgt_p[t], F.R, ts_c, ts_g, false);
-        for (int f = 0; f < F.nf; ++f) sp.spread[f] = (2.0 * M_PI) * ts_c / seq_value(F.freq, f);
-        for (int f = 0; f < F.nf; ++f) sp.spread[f] = sp.spread[f] * sp.spread[f];
+        for (int k = 0; k < kSlots; ++k) {
+            if (k >= fs.count()) break;
+            const double w = (2.0 * M_PI) * ts_c / seq_value(F.freq, fs.freq(k));
+            sp.spread[k] = w * w;
+        }
         {
             int a = az;
             double no[3] = {offset[0], offset[1], offset[2]};
@@ -1412,27 +1459,27 @@ WQ_HD void body_build(int64_t c, const DEnv& E, const FanConst& F, const Arrays&
             }
             intensity_az(sp, d, az, no, distance);
         }
-        double inten[kMaxFreq];
-        for (int f = 0; f < F.nf; ++f)
-            inten[f] = -10.0 * wqm::log10(smax(sp.I_de[f] * sp.I_az[f], 1e-30));
+        double inten[kSlots];
         double dt = offset[0] / F.dt;
-        if (dt >= 0.0) {
-            for (int f = 0; f < F.nf; ++f) {
+        const bool fwd = dt >= 0.0;
+        if (!fwd) dt = 1.0 + dt;
+        bool above = false;
+        for (int k = 0; k < kSlots; ++k) {
+            if (k >= fs.count()) break;
+            const int f = fs.freq(k);
+            inten[k] = -10.0 * wqm::log10(smax(sp.I_de[k] * sp.I_az[k], 1e-30));
+            if (fwd) {
                 const double ac = A.atten[((int64_t)S.curr * F.nf + f) * F.N + i];
                 const double an = A.atten[((int64_t)S.next * F.nf + f) * F.N + i];
-                inten[f] = inten[f] + ac * (1.0 - dt) + an * dt;
-            }
-        } else {
-            dt = 1.0 + dt;
-            for (int f = 0; f < F.nf; ++f) {
+                inten[k] = inten[k] + ac * (1.0 - dt) + an * dt;
+            } else {
                 const double ap = A.atten[((int64_t)S.prev * F.nf + f) * F.N + i];
                 const double ac = A.atten[((int64_t)S.curr * F.nf + f) * F.N + i];
-                inten[f] = inten[f] + ap * (1.0 - dt) + ac * dt;
+                inten[k] = inten[k] + ap * (1.0 - dt) + ac * dt;
             }
+            if (inten[k] < F.thr_intensity) above = true;
         }
-        bool above = false;
-        for (int f = 0; f < F.nf; ++f)
-            if (inten[f] < F.thr_intensity) above = true;
+        if (Split) above = group_any(above, G);
         if (!above) return;
         double V[3][3][3], W[3][3][3];
         const int sl[3] = {S.prev, S.curr, S.next};
@@ -1459,29 +1506,36 @@ WQ_HD void body_build(int64_t c, const DEnv& E, const FanConst& F, const Arrays&
         make_taylor_coeff(W, delta, cen, g, H);
         prod3(H, offset, Ho);
         const double target_az = cen + inner3(g, offset) + 0.5 * inner3(offset, Ho);
-        const unsigned long long slot = atom_add(Rc.ray_count, 1ull);
+        unsigned long long slot = 0;
+        if (!Split || fsel == 0) slot = atom_add(Rc.ray_count, 1ull);
+        if (Split) slot = group_bcast(slot, G);
         if ((int64_t)slot >= Rc.ray_cap) {
-            atom_or(Rc.error, 1);
+            if (!Split || fsel == 0) atom_or(Rc.error, 1);
             return;
         }
-        wq3d_eigenray& r = Rc.rays[slot];
-        r.step = step_key;
-        r.t1 = (int32_t)(t / F.nt2);
-        r.t2 = (int32_t)(t % F.nt2);
-        r.de = de;
-        r.az = az;
-        r.travel_time = time + offset[0];
-        r.source_de = seq_value(F.de, de) + offset[1];
-        r.source_az = seq_value(F.az, az) + offset[2];
-        r.target_de = target_de;
-        r.target_az = target_az;
-        r.surface = A.srf[ic];
-        r.bottom = A.btm[ic];
-        r.caustic = A.cst[ic];
-        r.upper = A.upr[ic];
-        r.lower = A.lwr[ic];
-        for (int f = 0; f < F.nf; ++f) {
-            Rc.ray_int[slot * F.nf + f] = inten[f];
+        if (!Split || fsel == 0) {
+            wq3d_eigenray& r = Rc.rays[slot];
+            r.step = step_key;
+            r.t1 = (int32_t)(t / F.nt2);
+            r.t2 = (int32_t)(t % F.nt2);
+            r.de = de;
+            r.az = az;
+            r.travel_time = time + offset[0];
+            r.source_de = seq_value(F.de, de) + offset[1];
+            r.source_az = seq_value(F.az, az) + offset[2];
+            r.target_de = target_de;
+            r.target_az = target_az;
+            r.surface = A.srf[ic];
+            r.bottom = A.btm[ic];
+            r.caustic = A.cst[ic];
+            r.upper = A.upr[ic];
+            r.lower = A.lwr[ic];
+        }
+        for (int k = 0; k < kSlots; ++k) {
+            if (k >= fs.count()) break;
+            if (Split && k == 0 && fsel != 0) continue;  // frequency 0 is written by its own lane
+            const int f = fs.freq(k);
+            Rc.ray_int[slot * F.nf + f] = inten[k];
             Rc.ray_phase[slot * F.nf + f] = A.phase[((int64_t)S.curr * F.nf + f) * F.N + i];
         }
     }
