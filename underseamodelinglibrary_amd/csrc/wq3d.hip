@@ -210,11 +210,12 @@ inline hipError_t last_error() { return hipGetLastError(); }
 #else
 // device allocations are not zeroed on the GPU: fill with 0xff (NaN doubles,
 // -1 integers) so reads before writes show up in the CPU checks.
-// WQ3D_EMU_FILL=<hex byte> or "random" picks another fill.
+// WQ3D_EMU_FILL=<hex byte>, "random" or "none" picks another fill.
 inline hipError_t malloc(void** p, size_t n) {
     *p = std::malloc(n);
     if (!*p) return hipErrorOutOfMemory;
     const char* fill = std::getenv("WQ3D_EMU_FILL");
+    if (fill && std::strcmp(fill, "none") == 0) return hipSuccess;  // left uninitialised
     if (fill && std::strcmp(fill, "random") == 0) {
         static unsigned long long state = 0x9e3779b97f4a7c15ull;
         unsigned char* b = static_cast<unsigned char*>(*p);
