@@ -26,6 +26,10 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 W_TGT = 17.0              # FP64 ops per (target, ray, step) of the distance test
+# HBM bytes per k_detect launch at this workload: rocprofv3 FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 (factors from a 1 GiB device copy), means of 310 launches;
+# profiles/k_detect_single_fan_115k_rays_10k_targets.md holds the commands
+K_DETECT_TRAFFIC_BYTES = (53780.0 * 2.0 + 32875.0) * 1024.0
 KERNELS = ["k_init", "k_reflect", "k_advance", "k_detect", "k_build", "k_exact"]
 
 
@@ -102,7 +106,8 @@ def main():
     flops = W_TGT * sc.num_targets() * n_rays
     achieved = flops / (det_ms * 1e-3) / 1e12 if det_ms > 0 else 0.0
     out = {
-        "metric": "ray-steps/sec (plain per-fan kernel)",
+        "metric": "ray-steps/sec + eigenrays/sec on 1 MI355X, plain vs batched+adjoint at fixed tol",
+        "form": "plain per-fan kernels",
         "value": value,
         "unit": "ray-steps/s",
         "n_gpus": 1,
@@ -123,8 +128,10 @@ def main():
         "kernel_ms_per_launch": kernel_ms,
         "roofline": {"bound": "mfma", "kernel": "k_detect", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "note": "FP64 vector; 17 FP64 ops per (target, ray) pair per launch"},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": K_DETECT_TRAFFIC_BYTES,
+                     "note": "FP64 vector (no MFMA-shaped work); achieved = 17 FP64 ops per "
+                             "(target, ray) pair x pairs per launch / mean launch time; traffic in "
+                             "bytes per launch"},
     }
     q.close()
     if not args.no_cpu_baseline:
