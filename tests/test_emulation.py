@@ -113,19 +113,20 @@ def test_detect_prefilter_matches_exact_comparisons():
     """The linear-form pre-filter of the eigenray search never changes a
     decision: random rays and targets, plus targets bisected onto the exact
     boundary of every comparison of the pre-test, decided with and without
-    it (emulation-only test hook)."""
+    it, and no block of targets the block bound rejects holds a target inside
+    the window (emulation-only test hook)."""
     out = _run("""
         import ctypes as C
-        counts = (C.c_int64 * 4)()
+        counts = (C.c_int64 * 5)()
         fn = lib.wq3d_emu_check_detect
         fn.argtypes = [C.c_int64, C.c_int64, C.c_uint64, C.c_double, C.POINTER(C.c_int64)]
         fn.restype = C.c_int32
-        total = [0, 0, 0, 0]
+        total = [0, 0, 0, 0, 0]
         for seed, radius in ((1, 6371000.0), (2, 6378101.030201019), (3, 6371000.0)):
             assert fn(200, 200, seed, radius, counts) == 0
             total = [a + b for a, b in zip(total, counts)]
         print("counts", *total)
     """)
-    pairs, mismatches, window, passes = (int(x) for x in out.split("counts")[-1].split())
+    pairs, mismatches, window, passes, blocks = (int(x) for x in out.split("counts")[-1].split())
     assert mismatches == 0
-    assert pairs > 100000 and window > passes > 100
+    assert pairs > 100000 and window > passes > 100 and blocks > 1000

@@ -45,7 +45,6 @@ using namespace wq3d;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kTargetTile = 256;
 
 #ifndef WQ3D_EMULATE
 __global__ void __launch_bounds__(kBlock) k_init(DEnv E, FanConst F, Arrays A, Slots S) {
@@ -78,22 +77,37 @@ struct alignas(16) LinFeatures {
 };
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef __attribute__((address_space(4))) const LinFeatures ConstLinFeatures;  // constant address space
+typedef __attribute__((address_space(4))) const TargetBox ConstTargetBox;
 #else
 typedef const LinFeatures ConstLinFeatures;
+typedef const TargetBox ConstTargetBox;
 #endif
 
+// Rays of a wavefront form an 8 x 8 tile of the fan (DE x AZ), so their
+// pre-test windows cross the target field in nearby places and whole target
+// blocks can be skipped (block_window_ok) for all 64 lanes at once.
+constexpr int kTile = 8;
+
+__device__ __forceinline__ int64_t tile_ray(const FanConst& F, int64_t tile, int lane) {
+    const int64_t tiles_az = (F.naz + kTile - 1) / kTile;
+    const int64_t de = (tile / tiles_az) * kTile + lane / kTile;
+    const int64_t az = (tile % tiles_az) * kTile + lane % kTile;
+    return (de < F.nde && az < F.naz) ? de * F.naz + az : -1;
+}
+
 __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S, Records Rc,
-                                                  int64_t tgt_per_group) {
+                                                  int64_t blk_per_group) {
     __shared__ uint32_t s_q[kQueue][kBlock];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t tile = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int64_t i = tile_ray(F, tile, lane);
     DetectWindow W;
     const bool active = detect_ray_window(F, A, S, i, W);
     if (!__any(active)) return;  // wave-uniform: no barriers below
-    const int64_t t_begin = (int64_t)blockIdx.y * tgt_per_group;
-    const int64_t t_end = (F.nt < t_begin + tgt_per_group) ? F.nt : t_begin + tgt_per_group;
-    const int lane = threadIdx.x & 63;
+    const int64_t b_begin = (int64_t)blockIdx.y * blk_per_group;
+    const int64_t b_end = (F.nblk < b_begin + blk_per_group) ? F.nblk : b_begin + blk_per_group;
     int qn = 0;
-    // write this wavefront's queued hits to the global list
+    // write this wavefront's queued hits (scan positions) to the global list
     auto flush = [&]() {
         int incl = qn;
         for (int off = 1; off < 64; off <<= 1) {
@@ -116,26 +130,33 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
         qn = 0;
     };
     const ConstLinFeatures* cf = (const ConstLinFeatures*)(F.tgt_lin);
-    auto visit = [&](const LinFeatures& fc, int64_t t) {
-        if (active && detect_window_ok(W, fc.f)) {
-            s_q[qn][threadIdx.x] = (uint32_t)t;
-            ++qn;
+    const ConstTargetBox* boxes = (const ConstTargetBox*)(F.tgt_box);
+    for (int64_t b = b_begin; b < b_end; ++b) {
+        const TargetBox box = boxes[b];
+        const bool need = active && block_window_ok(W, box);
+        if (!__any(need)) continue;
+        auto visit = [&](const LinFeatures& fc, int64_t t) {
+            if (need && detect_window_ok(W, fc.f)) {
+                s_q[qn][threadIdx.x] = (uint32_t)t;
+                ++qn;
+            }
+        };
+        // groups of four targets: one batch of scalar loads, at most four
+        // queue entries (the queues are flushed while they have room)
+        const int64_t t_end = (F.nt < (b + 1) * kTargetBlock) ? F.nt : (b + 1) * kTargetBlock;
+        int64_t t = b * kTargetBlock;
+        for (; t + 4 <= t_end; t += 4) {
+            const LinFeatures f0 = cf[t], f1 = cf[t + 1], f2 = cf[t + 2], f3 = cf[t + 3];
+            visit(f0, t);
+            visit(f1, t + 1);
+            visit(f2, t + 2);
+            visit(f3, t + 3);
+            if (__any(qn > kQueue - 4)) flush();
         }
-    };
-    // groups of four targets: one batch of scalar loads, at most four queue
-    // entries; the queues are flushed while they have room for a group
-    int64_t t = t_begin;
-    for (; t + 4 <= t_end; t += 4) {
-        const LinFeatures f0 = cf[t], f1 = cf[t + 1], f2 = cf[t + 2], f3 = cf[t + 3];
-        visit(f0, t);
-        visit(f1, t + 1);
-        visit(f2, t + 2);
-        visit(f3, t + 3);
-        if (__any(qn > kQueue - 4)) flush();
-    }
-    for (; t < t_end; ++t) {
-        visit(cf[t], t);
-        if (__any(qn == kQueue)) flush();
+        for (; t < t_end; ++t) {
+            visit(cf[t], t);
+            if (__any(qn == kQueue)) flush();
+        }
     }
     flush();
 }
@@ -150,9 +171,10 @@ __global__ void __launch_bounds__(kBlock) k_exact(FanConst F, Arrays A, Slots S,
     const int lane = threadIdx.x & 63;
     for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < nhit;
          h += (int64_t)gridDim.x * blockDim.x) {
-        const unsigned long long key = Rc.hit[h];
-        const int64_t t = (int64_t)(key >> 32);
-        const int64_t i = (int64_t)(key & 0xffffffffull);
+        const unsigned long long hk = Rc.hit[h];
+        const int64_t t = (int64_t)F.tgt_perm[hk >> 32];
+        const int64_t i = (int64_t)(hk & 0xffffffffull);
+        const unsigned long long key = ((unsigned long long)t << 32) | (unsigned long long)i;
         DetectRay r;
         const bool pass = detect_ray(F, A, S, i, r) && detect_pair(r, pts, dtarget(F, t), F.tgt_branch[t] != 0);
         const unsigned long long mask = __ballot(pass);
@@ -303,16 +325,21 @@ void launch_detect(hipStream_t st, unsigned grid, unsigned groups, const FanCons
 #ifndef WQ3D_EMULATE
     hipLaunchKernelGGL(k_detect, dim3(grid, groups), dim3(kBlock), 0, st, F, A, S, R, per_group);
 #else
+    const TargetBox* boxes = reinterpret_cast<const TargetBox*>(F.tgt_box);
     for (int64_t i = 0; i < F.N; ++i) {
         DetectWindow W;
         if (!detect_ray_window(F, A, S, i, W)) continue;
-        for (int64_t t = 0; t < F.nt; ++t) {
-            if (!detect_window_ok(W, F.tgt_lin + t * 6)) continue;
-            const unsigned long long off = (*R.hit_count)++;
-            if ((int64_t)off < R.hit_cap)
-                R.hit[off] = ((unsigned long long)t << 32) | (unsigned long long)i;
-            else
-                *R.error |= 2;
+        for (int64_t b = 0; b < F.nblk; ++b) {
+            if (!block_window_ok(W, boxes[b])) continue;
+            const int64_t t_end = std::min<int64_t>(F.nt, (b + 1) * kTargetBlock);
+            for (int64_t t = b * kTargetBlock; t < t_end; ++t) {
+                if (!detect_window_ok(W, F.tgt_lin + t * 6)) continue;
+                const unsigned long long off = (*R.hit_count)++;
+                if ((int64_t)off < R.hit_cap)
+                    R.hit[off] = ((unsigned long long)t << 32) | (unsigned long long)i;
+                else
+                    *R.error |= 2;
+            }
         }
     }
 #endif
@@ -328,9 +355,9 @@ void launch_exact(hipStream_t st, const FanConst& F, const Arrays& A, const Slot
         std::fprintf(stderr, "[emu] hits %lld (cap %lld), candidates before %llu\n", (long long)*R.hit_count,
                      (long long)R.hit_cap, *R.cand_count);
     for (int64_t h = 0; h < n; ++h) {
-        const unsigned long long key = R.hit[h];
-        const int64_t t = (int64_t)(key >> 32);
-        const int64_t i = (int64_t)(key & 0xffffffffull);
+        const int64_t t = (int64_t)F.tgt_perm[R.hit[h] >> 32];
+        const int64_t i = (int64_t)(R.hit[h] & 0xffffffffull);
+        const unsigned long long key = ((unsigned long long)t << 32) | (unsigned long long)i;
         DetectRay r;
         if (!detect_ray(F, A, S, i, r) || !detect_pair(r, pts, dtarget(F, t), F.tgt_branch[t] != 0)) continue;
         const unsigned long long off = (*R.cand_count)++;
@@ -486,6 +513,29 @@ void target_features(double R, double at, double bt, double st, double f6[6]) {
     f6[4] = Rs * bt;
     f6[5] = Rs * (bt * bt);
 }
+
+// bounding box of a block of targets (see TargetBox)
+struct BoxAcc {
+    double alo = HUGE_VAL, ahi = -HUGE_VAL, blo = HUGE_VAL, bhi = -HUGE_VAL;
+    double rlo = HUGE_VAL, rhi = -HUGE_VAL, slo = HUGE_VAL, shi = -HUGE_VAL;
+    void add(double at, double bt, double r, double st) {
+        alo = std::min(alo, at);
+        ahi = std::max(ahi, at);
+        blo = std::min(blo, bt);
+        bhi = std::max(bhi, bt);
+        rlo = std::min(rlo, r);
+        rhi = std::max(rhi, r);
+        slo = std::min(slo, st);
+        shi = std::max(shi, st);
+    }
+    TargetBox box() const {
+        // centre and half width, the half width padded so that the centre's
+        // rounding cannot leave a target outside
+        return TargetBox{0.5 * (alo + ahi), 0.5 * (ahi - alo) * (1.0 + 1e-12) + 1e-300,
+                         0.5 * (blo + bhi), 0.5 * (bhi - blo) * (1.0 + 1e-12) + 1e-300,
+                         rlo, rhi, slo, shi};
+    }
+};
 
 // rounding margin of the linear forms for radii up to rmax: 2^-38 * 2 rho^2
 // with 20 km of headroom, at least 64 m^2
@@ -839,14 +889,42 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         }
         // linear-form features (see lin_coeffs) and their rounding margin:
         // 2^-38 * 2 rho^2 at the largest radius in play, at least 64 m^2
+        // scan order: targets sorted along a Morton curve of (theta, phi,
+        // rho) and cut into blocks of kTargetBlock, so a block is compact
+        std::vector<uint32_t> order(nt);
+        {
+            double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+            const double* src[3] = {d->target_theta, d->target_phi, d->target_rho};
+            for (int64_t t = 0; t < nt; ++t)
+                for (int c = 0; c < 3; ++c) {
+                    lo[c] = std::min(lo[c], src[c][t]);
+                    hi[c] = std::max(hi[c], src[c][t]);
+                }
+            std::vector<uint64_t> key(nt);
+            for (int64_t t = 0; t < nt; ++t) {
+                uint64_t k = 0;
+                uint32_t q[3];
+                for (int c = 0; c < 3; ++c) {
+                    const double span = hi[c] - lo[c];
+                    const double u = span > 0.0 ? (src[c][t] - lo[c]) / span : 0.0;
+                    q[c] = (uint32_t)std::min(1023.0, std::max(0.0, std::floor(u * 1024.0)));
+                }
+                for (int bit = 9; bit >= 0; --bit)
+                    for (int c = 0; c < 3; ++c) k = (k << 1) | ((q[c] >> bit) & 1u);
+                key[t] = (k << 32) | (uint64_t)t;
+            }
+            std::sort(key.begin(), key.end());
+            for (int64_t t = 0; t < nt; ++t) order[t] = (uint32_t)(key[t] & 0xffffffffull);
+        }
         std::vector<double> lin((size_t)nt * 6);
         double rmax = F.R;
         for (int j = 0; j < 6; ++j) {
             F.lin_fmin[j] = HUGE_VAL;
             F.lin_fmax[j] = -HUGE_VAL;
         }
-        for (int64_t t = 0; t < nt; ++t) {
-            double* f6 = &lin[(size_t)t * 6];
+        for (int64_t k = 0; k < nt; ++k) {
+            const uint32_t t = order[k];
+            double* f6 = &lin[(size_t)k * 6];
             target_features(d->target_rho[t], th[t], ph[t], ts[t], f6);
             rmax = std::max(rmax, std::fabs(d->target_rho[t]));
             for (int j = 0; j < 6; ++j) {
@@ -855,7 +933,19 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
             }
         }
         F.lin_margin = lin_margin(rmax);
+        F.nblk = (nt + kTargetBlock - 1) / kTargetBlock;
+        std::vector<TargetBox> box((size_t)std::max<int64_t>(1, F.nblk));
+        for (int64_t b = 0; b < F.nblk; ++b) {
+            BoxAcc acc;
+            for (int64_t k = b * kTargetBlock; k < std::min<int64_t>(nt, (b + 1) * kTargetBlock); ++k) {
+                const uint32_t t = order[k];
+                acc.add(th[t], ph[t], d->target_rho[t], ts[t]);
+            }
+            box[b] = acc.box();
+        }
         if (f->mem.upload(&F.tgt_lin, lin.data(), lin.size()) != hipSuccess ||
+            f->mem.upload(&F.tgt_perm, order.data(), order.size()) != hipSuccess ||
+            f->mem.upload(&F.tgt_box, reinterpret_cast<const double*>(box.data()), box.size() * 8) != hipSuccess ||
             f->mem.upload(&F.tgt_th, th.data(), nt) != hipSuccess ||
             f->mem.upload(&F.tgt_ph, ph.data(), nt) != hipSuccess ||
             f->mem.upload(&F.tgt_r2, r2.data(), nt) != hipSuccess ||
@@ -985,13 +1075,16 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
     const FanConst& F = f->F;
     const unsigned grid = (unsigned)((F.N + kBlock - 1) / kBlock);
     const DEnv& E = f->env->dev;
-    // targets are split into groups so that the detect grid has enough blocks
+    // the detect grid: 4 ray tiles (wavefronts) per block in x, target
+    // blocks split into groups in y so that the grid has enough blocks
+    const int64_t tiles = ((F.nde + 7) / 8) * ((F.naz + 7) / 8);
+    const unsigned dgrid = (unsigned)((tiles + 3) / 4);
     int64_t groups = 1;
     if (F.nt > 0) {
-        groups = std::max<int64_t>(1, (int64_t)4096 / std::max<unsigned>(1u, grid));
-        groups = std::min<int64_t>(groups, (F.nt + kTargetTile - 1) / kTargetTile);
+        groups = std::max<int64_t>(1, (int64_t)4096 / std::max<unsigned>(1u, dgrid));
+        groups = std::min<int64_t>(groups, std::max<int64_t>(1, F.nblk / 4));
     }
-    const int64_t per_group = F.nt > 0 ? (F.nt + groups - 1) / groups : 0;
+    const int64_t per_group = F.nblk > 0 ? (F.nblk + groups - 1) / groups : 0;
     for (int64_t s = 0; s < n; ++s) {
         // detect_reflections
         {
@@ -1021,7 +1114,7 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
             // candidate and hit counters are adjacent
             HIP_TRY(rt::memset_async(f->R.cand_count, 0, 2 * sizeof(unsigned long long), f->stream));
             HIP_TRY(timed(f, KID_DETECT, [&] {
-                launch_detect(f->stream, grid, (unsigned)groups, F, f->A, S, f->R, per_group);
+                launch_detect(f->stream, dgrid, (unsigned)groups, F, f->A, S, f->R, per_group);
             }));
             HIP_TRY(timed(f, KID_EXACT, [&] { launch_exact(f->stream, F, f->A, S, f->R); }));
             const double t = f->time;
@@ -1247,7 +1340,9 @@ wq3d_status wq3d_kernel_times(wq3d_fan* f, double* ms, int64_t* launches, int32_
 // the earth's radius, plus targets placed by bisection on the boundaries
 // where an exact comparison of the pre-test flips, decided once with the
 // linear-form pre-filter and once with the exact expressions alone.
-// counts = {pairs, mismatches, pairs inside the window, passes}.
+// Blocks of 8 consecutive generated targets check the block bound the same
+// way.  counts = {pairs, mismatches (pairs and rejected blocks), pairs
+// inside the window, passes, rejected blocks}.
 static double emu_uniform(unsigned long long& s, double lo, double hi) {
     s = s * 6364136223846793005ull + 1442695040888963407ull;
     return lo + (hi - lo) * ((double)(s >> 11) * 0x1.0p-53);
@@ -1268,7 +1363,7 @@ int32_t wq3d_emu_check_detect(int64_t num_rays, int64_t num_targets, uint64_t se
     unsigned long long s = seed * 2654435761ull + 1;
     const double R0 = earth_radius;
     const double deg = M_PI / 180.0;
-    for (int k = 0; k < 4; ++k) counts[k] = 0;
+    for (int k = 0; k < 5; ++k) counts[k] = 0;
     const double margin = lin_margin(R0);
     for (int64_t ray = 0; ray < num_rays; ++ray) {
         // ray state: position, velocity over one step (about 150 m), curvature
@@ -1368,6 +1463,21 @@ int32_t wq3d_emu_check_detect(int64_t num_rays, int64_t num_targets, uint64_t se
             }
         }
         const DetectWindow W = detect_window(F, lin_diff(fp.n, dr.cur), lin_diff(fp.p, dr.cur));
+        // blocks of 8 consecutive generated targets (the bisected ones sit
+        // together): a rejected block must hold no target inside the window
+        for (size_t b0 = 0; b0 < tg.size(); b0 += 8) {
+            BoxAcc acc;
+            const size_t b1 = std::min(tg.size(), b0 + 8);
+            for (size_t k = b0; k < b1; ++k) acc.add(0.5 * tg[k].t, 0.5 * tg[k].p, tg[k].r, wqm::sin(tg[k].t));
+            if (block_window_ok(W, acc.box())) continue;
+            counts[4] += 1;
+            for (size_t k = b0; k < b1; ++k) {
+                DTarget x;
+                double f6[6];
+                make(tg[k].r, tg[k].t, tg[k].p, x, f6);
+                counts[1] += detect_window_ok(W, f6) ? 1 : 0;
+            }
+        }
         for (const Tgt& g : tg) {
             DTarget x;
             double f6[6];

@@ -74,7 +74,13 @@ struct FanConst {
     const double *tgt_th, *tgt_ph, *tgt_r2, *tgt_2r;
     // [nt][6] target features of the distance expression written as a
     // linear form (see lin_coeffs) and the rounding margin of that form
+    // Targets in scan order: spatially sorted and cut into blocks of
+    // kTargetBlock.  tgt_lin holds their features, tgt_perm the target index
+    // of each scan position, tgt_box one TargetBox per block.
     const double* tgt_lin;
+    const uint32_t* tgt_perm;
+    const double* tgt_box;
+    int64_t nblk;
     double lin_margin;
     double lin_fmin[6], lin_fmax[6];  // feature ranges over all targets
     // spreading_hybrid_gaussian norms (spreading_hybrid_gaussian.cc:28-59)
@@ -802,6 +808,40 @@ WQ_HD_INLINE DetectWindow detect_window(const FanConst& F, const LinForm& next, 
 WQ_HD_INLINE bool detect_window_ok(const DetectWindow& W, const double f[6]) {
     if (fabs(lin_eval(W.form, f)) > W.w) return false;
     return !(lin_eval(W.prev, f) < -W.m);
+}
+
+// Box of one block of targets, in the variables of the features: theta/2
+// and phi/2 as centre and half width, rho and sin(theta) as ranges.
+constexpr int kTargetBlock = 64;
+struct TargetBox {
+    double at0, hx, bt0, hy, r_lo, r_hi, s_lo, s_hi;
+};
+
+// Interval of the window form over every target of a box.  With the
+// features written as R (P(at) + st Q(bt)), P and Q quadratics in theta/2
+// and phi/2, their Taylor expansions about the box centre bound P and Q
+// exactly; st and R are positive ranges.  false when no target of the box
+// can be inside the window (one more margin absorbs the rounding of this
+// bound and of the features).
+WQ_HD_INLINE bool block_window_ok(const DetectWindow& W, const TargetBox& B) {
+    const double* a = W.form.a;
+    const double P0 = a[0] + a[1] * B.at0 + a[2] * (B.at0 * B.at0);
+    const double P1 = fabs(a[1] + 2.0 * a[2] * B.at0) * B.hx;
+    const double P2 = a[2] * (B.hx * B.hx);
+    const double p_lo = P0 - P1 + (P2 < 0.0 ? P2 : 0.0);
+    const double p_hi = P0 + P1 + (P2 > 0.0 ? P2 : 0.0);
+    const double Q0 = a[3] + a[4] * B.bt0 + a[5] * (B.bt0 * B.bt0);
+    const double Q1 = fabs(a[4] + 2.0 * a[5] * B.bt0) * B.hy;
+    const double Q2 = a[5] * (B.hy * B.hy);
+    const double q_lo = Q0 - Q1 + (Q2 < 0.0 ? Q2 : 0.0);
+    const double q_hi = Q0 + Q1 + (Q2 > 0.0 ? Q2 : 0.0);
+    const double sq_lo = fmin(fmin(B.s_lo * q_lo, B.s_hi * q_lo), fmin(B.s_lo * q_hi, B.s_hi * q_hi));
+    const double sq_hi = fmax(fmax(B.s_lo * q_lo, B.s_hi * q_lo), fmax(B.s_lo * q_hi, B.s_hi * q_hi));
+    const double g_lo = p_lo + sq_lo, g_hi = p_hi + sq_hi;
+    const double lo = W.form.k + fmin(B.r_lo * g_lo, B.r_hi * g_lo);
+    const double hi = W.form.k + fmax(B.r_lo * g_hi, B.r_hi * g_hi);
+    const double w = W.w + W.m;
+    return !(lo > w || hi < -w);
 }
 
 // Ray points of the exact stage, read from the state arrays on demand.
