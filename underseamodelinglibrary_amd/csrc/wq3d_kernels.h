@@ -1181,6 +1181,34 @@ WQ_HD void compute_offsets(const BuildCtx& b, int64_t t, int de, int az, const d
 }
 
 // spreading_hybrid_gaussian (spreading_hybrid_gaussian.cc:20-571)
+// Lane-group helpers of the split build: a candidate's lanes are
+// consecutive lanes of one wavefront, starting at a multiple of G.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ bool group_any(bool p, int G) {
+    const unsigned long long m = __ballot(p);
+    const int base = (int)(threadIdx.x & 63) & ~(G - 1);
+    const unsigned long long sel = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << base);
+    return (m & sel) != 0ull;
+}
+__device__ __forceinline__ double group_shfl(double v, int src, int G) {
+    const int lane = (int)(threadIdx.x & 63) & ~(G - 1);
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned int lo = (unsigned int)__shfl((int)(unsigned int)(u & 0xffffffffull), lane + src);
+    const unsigned int hi = (unsigned int)__shfl((int)(unsigned int)(u >> 32), lane + src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ unsigned long long group_bcast(unsigned long long v, int G) {
+    const int base = (int)(threadIdx.x & 63) & ~(G - 1);
+    const unsigned int lo = (unsigned int)__shfl((int)(unsigned int)(v & 0xffffffffull), base);
+    const unsigned int hi = (unsigned int)__shfl((int)(unsigned int)(v >> 32), base);
+    return ((unsigned long long)hi << 32) | lo;
+}
+#else
+inline bool group_any(bool p, int) { return p; }
+inline unsigned long long group_bcast(unsigned long long v, int) { return v; }
+inline double group_shfl(double v, int, int) { return v; }
+#endif
+
 // Frequencies one thread evaluates in body_build.  Split: the candidate is
 // built by a group of lanes, lane g evaluating frequency 0 (which steers the
 // spreading walks) and frequency g, in two register slots.  Not split: one
@@ -1189,7 +1217,8 @@ template <bool Split>
 struct FreqSel {
     static constexpr int kSlots = Split ? 2 : kMaxFreq;
     int nf;
-    int fsel;  // Split only: this lane's frequency (0 .. nf-1)
+    int fsel;  // Split only: this lane's frequency (0 .. nf-1), its lane in the group
+    int G;     // Split only: lanes in the group
     WQ_HD_INLINE int count() const { return Split ? (fsel > 0 ? 2 : 1) : nf; }
     WQ_HD_INLINE int freq(int k) const { return Split ? (k == 0 ? 0 : fsel) : k; }
 };
@@ -1267,6 +1296,34 @@ WQ_HD double width_az(const BuildCtx& b, int de, int az, const double offset[3])
     return 0.5 * ((1.0 - u) * length1 + u * length2);
 }
 
+// Widths of the cells one spreading walk visits, cell k of the walk being
+// cell(k).  A split build computes them G at a time, lane g taking cell
+// base + g, and hands each out by a shuffle; the widths do not depend on
+// the walk's state, so computing a few past the walk's end changes nothing.
+// A plain build computes each width when it is asked for.
+template <bool Split, class WidthOf>
+struct WidthStream {
+    WidthOf width_of;  // width of cell k of the walk; any k >= 0 must be safe
+    int G, g;
+    int base = 0;
+    double w = 0.0;
+    bool loaded = false;
+    WQ_HD_INLINE double get(int k) {
+        if (!Split) return width_of(k);
+        if (!loaded || k >= base + G) {
+            base = k;
+            w = width_of(k + g);
+            loaded = true;
+        }
+        return group_shfl(w, k - base, G);
+    }
+};
+
+template <bool Split, class WidthOf>
+WQ_HD_INLINE WidthStream<Split, WidthOf> width_stream(const FreqSel<Split>& fs, WidthOf fn) {
+    return WidthStream<Split, WidthOf>{fn, fs.G, fs.fsel};
+}
+
 // spreading_hybrid_gaussian::gaussian (.h:148-153), accumulated into out
 // (slot k holds frequency s.fs.freq(k))
 template <bool Split>
@@ -1300,15 +1357,17 @@ WQ_HD void intensity_de(Spread<Split>& s, int de, int az, const double offset[3]
     cell_dist = L + cell_width;
     gaussian_add(s, cell_dist, cell_width, norm_de[d], s.I_de, false);
     if (s.I_de[0] < 1e-10) return;
+    auto down = width_stream(s.fs, [&](int k) { return de - 2 - k >= 0 ? width_de(b, de - 2 - k, az, offset) : 0.0; });
     for (d = de - 2; d >= 0; --d) {
         const bool virtual_ray = edge_at(b, d + 1, az) && edge_at(b, d, az);
         double nn;
         cell_dist += cell_width;
+        const double wd = down.get(de - 2 - d);
         if (virtual_ray) {
             cell_dist += cell_width;
             nn = norm_de[d + 1];
         } else {
-            cell_width = width_de(b, d, az, offset);
+            cell_width = wd;
             cell_dist += cell_width;
             nn = norm_de[d];
         }
@@ -1320,15 +1379,17 @@ WQ_HD void intensity_de(Spread<Split>& s, int de, int az, const double offset[3]
     cell_width = initial_width;
     cell_dist = L - cell_width;
     const int size = b.F.nde - 1;
+    auto up = width_stream(s.fs, [&](int k) { return de + 1 + k < size ? width_de(b, de + 1 + k, az, offset) : 0.0; });
     for (d = de + 1; d < size; ++d) {
         const bool virtual_ray = edge_at(b, d + 1, az) && edge_at(b, d, az);
         double nn;
         cell_dist -= cell_width;
+        const double wd = up.get(d - de - 1);
         if (virtual_ray) {
             cell_dist -= cell_width;
             nn = norm_de[d - 1];
         } else {
-            cell_width = width_de(b, d, az, offset);
+            cell_width = wd;
             cell_dist -= cell_width;
             nn = norm_de[d];
         }
@@ -1385,11 +1446,18 @@ WQ_HD void intensity_az(Spread<Split>& s, int de, int az, const double offset[3]
     } else {
         --a;
     }
+    // the walk visits a, a-1, ... wrapping from 0 to max_az-1
+    const int a_down = a;
+    auto down = width_stream(s.fs, [&](int k) {
+        const int c = ((a_down - k) % max_az + max_az) % max_az;
+        return width_az(b, de, c, offset);
+    });
+    int kd = 0;
     while ((a % max_az) != az_lower) {
         if (dup_test_set(s, a)) break;
         if (edge_at(b, de, a)) break;
         cell_dist += cell_width;
-        cell_width = width_az(b, de, a, offset);
+        cell_width = down.get(kd++);
         cell_dist += cell_width;
         const double old_tl = s.I_az[0];
         gaussian_add(s, cell_dist, cell_width, norm(de, a), s.I_az, false);
@@ -1403,12 +1471,15 @@ WQ_HD void intensity_az(Spread<Split>& s, int de, int az, const double offset[3]
     cell_width = initial_width;
     cell_dist = L - cell_width;
     a = az + 1;
+    // the walk visits az+1, az+2, ... modulo max_az
+    auto up = width_stream(s.fs, [&](int k) { return width_az(b, de, (az + 1 + k) % max_az, offset); });
+    int ku = 0;
     while ((a % max_az) != az_upper) {
         if (a == max_az) a = 0;
         if (dup_test_set(s, a)) break;
         if (edge_at(b, de, a)) break;
         cell_dist -= cell_width;
-        cell_width = width_az(b, de, a, offset);
+        cell_width = up.get(ku++);
         cell_dist -= cell_width;
         const double old_tl = s.I_az[0];
         gaussian_add(s, cell_dist, cell_width, norm(de, a), s.I_az, false);
@@ -1424,26 +1495,6 @@ WQ_HD_INLINE void ndir_angles(const Arrays& A, int64_t k, double& de, double& az
     az = to_degrees(wqm::atan2(np, -nt));
 }
 
-// Lane-group helpers of the split build: a candidate's lanes are
-// consecutive lanes of one wavefront, starting at a multiple of G.
-#if defined(__HIP_DEVICE_COMPILE__)
-__device__ __forceinline__ bool group_any(bool p, int G) {
-    const unsigned long long m = __ballot(p);
-    const int base = (int)(threadIdx.x & 63) & ~(G - 1);
-    const unsigned long long sel = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << base);
-    return (m & sel) != 0ull;
-}
-__device__ __forceinline__ unsigned long long group_bcast(unsigned long long v, int G) {
-    const int base = (int)(threadIdx.x & 63) & ~(G - 1);
-    const unsigned int lo = (unsigned int)__shfl((int)(unsigned int)(v & 0xffffffffull), base);
-    const unsigned int hi = (unsigned int)__shfl((int)(unsigned int)(v >> 32), base);
-    return ((unsigned long long)hi << 32) | lo;
-}
-#else
-inline bool group_any(bool p, int) { return p; }
-inline unsigned long long group_bcast(unsigned long long v, int) { return v; }
-#endif
-
 // is_closest_ray + build_eigenray (wave_queue.cc:411-702) for candidate c.
 // Split: called by the G lanes of a group with fsel = the lane's frequency;
 // every lane runs the same control flow (the walks stop on frequency 0,
@@ -1452,7 +1503,7 @@ template <bool Split>
 WQ_HD void body_build(int64_t c, int fsel, int G, const DEnv& E, const FanConst& F, const Arrays& A,
                       const Slots& S, const Records& Rc, double time, int64_t step_key) {
     const BuildCtx b{E, F, A, S};
-    const FreqSel<Split> fs{F.nf, fsel};
+    const FreqSel<Split> fs{F.nf, fsel, G};
     constexpr int kSlots = FreqSel<Split>::kSlots;
     {
         const unsigned long long key = Rc.cand[c];
