@@ -86,6 +86,15 @@ def test_eigenray_concave_parity(oracle_mod):
     set_earth_radius(6371000.0)
 
 
+def test_five_frequency_parity(oracle_mod):
+    """Config 1 at full size with 5 frequencies: the eigenray construction
+    runs on groups of 8 lanes of which 3 take no part."""
+    sc = scenarios.config1()
+    sc.freq = seq_log(250.0, 2.0, 5)
+    g, o = _run(sc, oracle_mod)
+    assert _check_all(g, o, sc.freq.size()) > 1000
+
+
 def test_multifrequency_fan_parity(oracle_mod):
     """The benchmark's frequencies (8 log-spaced, 100-10000 Hz) and
     environment on a thinned fan and a 20 x 20 target grid over 30 s."""

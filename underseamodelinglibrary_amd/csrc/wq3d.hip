@@ -200,11 +200,14 @@ __global__ void __launch_bounds__(kBlock) k_build(DEnv E, FanConst F, Arrays A, 
                                                  double time, int64_t step_key, int group) {
     const unsigned long long raw = *Rc.cand_count;
     const int64_t ncand = (int64_t)(raw < (unsigned long long)Rc.cand_cap ? raw : (unsigned long long)Rc.cand_cap);
+    // 27 neighbourhood values (or 2 x 27) per group of at least 4 lanes
+    __shared__ double s_cube[(kBlock / 4) * 54];
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane_in_group = (int)(gid % group);
+    double* lds = group >= 4 ? s_cube + (threadIdx.x / group) * 54 : nullptr;
     if (lane_in_group >= F.nf) return;  // lanes beyond the fan's frequencies idle
     for (int64_t c = gid / group; c < ncand; c += (int64_t)gridDim.x * blockDim.x / group)
-        body_build<true>(c, lane_in_group, group, E, F, A, S, Rc, time, step_key);
+        body_build<true>(c, lane_in_group, group, lds, E, F, A, S, Rc, time, step_key);
 }
 #endif  // WQ3D_EMULATE
 
@@ -378,7 +381,7 @@ void launch_build(hipStream_t st, const DEnv& E, const FanConst& F, const Arrays
     hipLaunchKernelGGL(k_build, dim3(1024), dim3(kBlock), 0, st, E, F, A, S, R, t, key, group);
 #else
     const int64_t n = (int64_t)std::min<unsigned long long>(*R.cand_count, (unsigned long long)R.cand_cap);
-    for (int64_t c = 0; c < n; ++c) body_build<false>(c, -1, 1, E, F, A, S, R, t, key);
+    for (int64_t c = 0; c < n; ++c) body_build<false>(c, -1, 1, nullptr, E, F, A, S, R, t, key);
 #endif
 }
 

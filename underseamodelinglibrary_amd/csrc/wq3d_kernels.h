@@ -953,28 +953,35 @@ WQ_HD_INLINE double dist2_slot(const BuildCtx& b, int slot, int64_t i, int64_t t
                             b.F.tgt_p[t], b.F.tgt_s[t]);
 }
 
-// wave_queue::is_closest_ray (wave_queue.cc:411-488)
-WQ_HD bool is_closest_ray(const BuildCtx& b, int64_t t, int de, int az, double center,
-                               double D[3][3][3]) {
+// Ray of the 3 x 3 neighbourhood point (nde, naz) of ray (de, az), with the
+// AZ wrap of is_closest_ray and build_eigenray (wave_queue.cc:423-431,
+// :544-548).
+WQ_HD_INLINE int64_t cube_ray(const FanConst& F, int de, int az, int nde, int naz) {
+    const int max_az = F.naz - 1;
+    const int d = de + nde - 1;
+    int a = az + naz - 1;
+    if (F.az_boundary) {
+        if (az + naz == 0) {
+            a = F.naz - 2;
+        } else if (a >= max_az) {
+            a = 0;
+        }
+    }
+    return (int64_t)d * F.naz + a;
+}
+
+// wave_queue::is_closest_ray (wave_queue.cc:411-488) on the distances D of
+// the whole (time, DE, AZ) neighbourhood, checked in the reference's order.
+WQ_HD bool is_closest_ray(const BuildCtx& b, int64_t t, int de, int az, const double D[3][3][3]) {
     const FanConst& F = b.F;
     const int max_az = F.naz - 1;
     const bool branch = F.tgt_branch[t] != 0;
+    const double center = D[1][1][1];
     for (int nde = 0; nde < 3; ++nde) {
         for (int naz = 0; naz < 3; ++naz) {
             if (nde == 1 && naz == 1) continue;
-            const int d = de + nde - 1;
-            int a = az + naz - 1;
-            if (F.az_boundary) {
-                if (az + naz == 0) {
-                    a = F.naz - 2;
-                } else if (a >= max_az) {
-                    a = 0;
-                }
-            }
-            const int64_t i = (int64_t)d * F.naz + a;
-            D[0][nde][naz] = dist2_slot(b, b.S.prev, i, t);
-            D[1][nde][naz] = dist2_slot(b, b.S.curr, i, t);
-            D[2][nde][naz] = dist2_slot(b, b.S.next, i, t);
+            const int64_t i = cube_ray(F, de, az, nde, naz);
+            const int a = (int)(i % F.naz);
             if (a == 0 && !F.az_boundary) continue;
             if (a == max_az) continue;
             const bool edge = b.A.edge[at(b.S.curr, F.N, i)] != 0;
@@ -1218,7 +1225,8 @@ struct FreqSel {
     static constexpr int kSlots = Split ? 2 : kMaxFreq;
     int nf;
     int fsel;  // Split only: this lane's frequency (0 .. nf-1), its lane in the group
-    int G;     // Split only: lanes in the group
+    int G;     // Split only: lanes in the group; lanes nf .. G-1 take no part
+    WQ_HD_INLINE int lanes() const { return nf < G ? nf : G; }  // lanes that take part
     WQ_HD_INLINE int count() const { return Split ? (fsel > 0 ? 2 : 1) : nf; }
     WQ_HD_INLINE int freq(int k) const { return Split ? (k == 0 ? 0 : fsel) : k; }
 };
@@ -1297,20 +1305,21 @@ WQ_HD double width_az(const BuildCtx& b, int de, int az, const double offset[3])
 }
 
 // Widths of the cells one spreading walk visits, cell k of the walk being
-// cell(k).  A split build computes them G at a time, lane g taking cell
-// base + g, and hands each out by a shuffle; the widths do not depend on
-// the walk's state, so computing a few past the walk's end changes nothing.
-// A plain build computes each width when it is asked for.
+// cell(k).  A split build computes them L at a time (L = the lanes taking
+// part), lane g taking cell base + g, and hands each out by a shuffle; the
+// widths do not depend on the walk's state, so computing a few past the
+// walk's end changes nothing.  A plain build computes each width when it is
+// asked for.
 template <bool Split, class WidthOf>
 struct WidthStream {
     WidthOf width_of;  // width of cell k of the walk; any k >= 0 must be safe
-    int G, g;
+    int G, L, g;
     int base = 0;
     double w = 0.0;
     bool loaded = false;
     WQ_HD_INLINE double get(int k) {
         if (!Split) return width_of(k);
-        if (!loaded || k >= base + G) {
+        if (!loaded || k >= base + L) {
             base = k;
             w = width_of(k + g);
             loaded = true;
@@ -1321,7 +1330,34 @@ struct WidthStream {
 
 template <bool Split, class WidthOf>
 WQ_HD_INLINE WidthStream<Split, WidthOf> width_stream(const FreqSel<Split>& fs, WidthOf fn) {
-    return WidthStream<Split, WidthOf>{fn, fs.G, fs.fsel};
+    return WidthStream<Split, WidthOf>{fn, fs.G, fs.lanes(), fs.fsel};
+}
+
+// Orders the LDS writes of a wavefront's lanes before their reads by other
+// lanes of the same wavefront.
+WQ_HD_INLINE void wave_sync() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#endif
+}
+
+// Values at the 27 points k = s*9 + nde*3 + naz of a (time, DE, AZ)
+// neighbourhood.  A split build with an LDS area computes them a lane at a
+// time (lane g takes k = g, g + L, ...) and every lane reads all 27 back;
+// otherwise one thread computes them all.  fn(k, out, stride) writes out[0]
+// and, for two values per point, out[stride].
+template <bool Split, class PointFn>
+WQ_HD_INLINE void fill_cube(const FreqSel<Split>& fs, double* lds, int nval, double* out, PointFn fn) {
+    if (Split && lds != nullptr) {
+        for (int k = fs.fsel; k < 27; k += fs.lanes()) fn(k, lds + k, 27);
+        wave_sync();
+        for (int k = 0; k < 27 * nval; ++k) out[k] = lds[k];
+        wave_sync();
+    } else {
+        for (int k = 0; k < 27; ++k) fn(k, out + k, 27);
+    }
 }
 
 // spreading_hybrid_gaussian::gaussian (.h:148-153), accumulated into out
@@ -1500,8 +1536,8 @@ WQ_HD_INLINE void ndir_angles(const Arrays& A, int64_t k, double& de, double& az
 // every lane runs the same control flow (the walks stop on frequency 0,
 // which every lane evaluates), so the group stays converged.
 template <bool Split>
-WQ_HD void body_build(int64_t c, int fsel, int G, const DEnv& E, const FanConst& F, const Arrays& A,
-                      const Slots& S, const Records& Rc, double time, int64_t step_key) {
+WQ_HD void body_build(int64_t c, int fsel, int G, double* lds, const DEnv& E, const FanConst& F,
+                      const Arrays& A, const Slots& S, const Records& Rc, double time, int64_t step_key) {
     const BuildCtx b{E, F, A, S};
     const FreqSel<Split> fs{F.nf, fsel, G};
     constexpr int kSlots = FreqSel<Split>::kSlots;
@@ -1510,12 +1546,12 @@ WQ_HD void body_build(int64_t c, int fsel, int G, const DEnv& E, const FanConst&
         const int64_t t = (int64_t)(key >> 32);
         const int64_t i = (int64_t)(key & 0xffffffffull);
         const int de = (int)(i / F.naz), az = (int)(i % F.naz);
+        const int sl[3] = {S.prev, S.curr, S.next};
         double D[3][3][3];
-        const double center = dist2_slot(b, S.curr, i, t);
-        D[1][1][1] = center;
-        D[2][1][1] = dist2_slot(b, S.next, i, t);
-        D[0][1][1] = dist2_slot(b, S.prev, i, t);
-        if (!is_closest_ray(b, t, de, az, center, D)) return;
+        fill_cube(fs, lds, 1, &D[0][0][0], [&](int k, double* out, int) {
+            out[0] = dist2_slot(b, sl[k / 9], cube_ray(F, de, az, (k / 3) % 3, k % 3), t);
+        });
+        if (!is_closest_ray(b, t, de, az, D)) return;
         // build_eigenray (wave_queue.cc:495-702)
         const int64_t ic = at(S.curr, F.N, i);
         if (above_bounce(F, A, ic)) return;
@@ -1572,24 +1608,12 @@ WQ_HD void body_build(int64_t c, int fsel, int G, const DEnv& E, const FanConst&
         }
         if (Split) above = group_any(above, G);
         if (!above) return;
-        double V[3][3][3], W[3][3][3];
-        const int sl[3] = {S.prev, S.curr, S.next};
-        const int max_az = F.naz - 1;
-        for (int nde = 0; nde < 3; ++nde) {
-            for (int naz = 0; naz < 3; ++naz) {
-                const int d = de + nde - 1;
-                int a = az + naz - 1;
-                if (F.az_boundary) {
-                    if (az + naz == 0) {
-                        a = max_az - 1;
-                    } else if (a >= max_az) {
-                        a = 0;
-                    }
-                }
-                const int64_t j = (int64_t)d * F.naz + a;
-                for (int s = 0; s < 3; ++s) ndir_angles(A, at(sl[s], F.N, j), V[s][nde][naz], W[s][nde][naz]);
-            }
-        }
+        double VW[2][3][3][3];  // launch-angle DE and AZ of the neighbourhood's directions
+        fill_cube(fs, lds, 2, &VW[0][0][0][0], [&](int k, double* out, int stride) {
+            ndir_angles(A, at(sl[k / 9], F.N, cube_ray(F, de, az, (k / 3) % 3, k % 3)), out[0], out[stride]);
+        });
+        const double(&V)[3][3][3] = VW[0];
+        const double(&W)[3][3][3] = VW[1];
         double cen, g[3], H[3][3], Ho[3];
         make_taylor_coeff(V, delta, cen, g, H);
         prod3(H, offset, Ho);
