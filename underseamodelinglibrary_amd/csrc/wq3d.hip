@@ -45,6 +45,7 @@ using namespace wq3d;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kBuildLanes = 32;  // lanes per eigenray candidate in k_build
 
 #ifndef WQ3D_EMULATE
 __global__ void __launch_bounds__(kBlock) k_init(DEnv E, FanConst F, Arrays A, Slots S) {
@@ -205,7 +206,6 @@ __global__ void __launch_bounds__(kBlock) k_build(DEnv E, FanConst F, Arrays A, 
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane_in_group = (int)(gid % group);
     double* lds = group >= 4 ? s_cube + (threadIdx.x / group) * 54 : nullptr;
-    if (lane_in_group >= F.nf) return;  // lanes beyond the fan's frequencies idle
     for (int64_t c = gid / group; c < ncand; c += (int64_t)gridDim.x * blockDim.x / group)
         body_build<true>(c, lane_in_group, group, lds, E, F, A, S, Rc, time, step_key);
 }
@@ -376,7 +376,7 @@ void launch_exact(hipStream_t st, const FanConst& F, const Arrays& A, const Slot
 void launch_build(hipStream_t st, const DEnv& E, const FanConst& F, const Arrays& A, const Slots& S,
                   const Records& R, double t, int64_t key) {
 #ifndef WQ3D_EMULATE
-    int group = 1;
+    int group = kBuildLanes;  // a power of two, at least the frequency count
     while (group < F.nf) group <<= 1;
     hipLaunchKernelGGL(k_build, dim3(1024), dim3(kBlock), 0, st, E, F, A, S, R, t, key, group);
 #else

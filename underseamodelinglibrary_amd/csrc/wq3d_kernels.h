@@ -1224,10 +1224,11 @@ template <bool Split>
 struct FreqSel {
     static constexpr int kSlots = Split ? 2 : kMaxFreq;
     int nf;
-    int fsel;  // Split only: this lane's frequency (0 .. nf-1), its lane in the group
-    int G;     // Split only: lanes in the group; lanes nf .. G-1 take no part
-    WQ_HD_INLINE int lanes() const { return nf < G ? nf : G; }  // lanes that take part
-    WQ_HD_INLINE int count() const { return Split ? (fsel > 0 ? 2 : 1) : nf; }
+    int fsel;  // Split only: this lane's index in the group; its frequency if below nf
+    int G;     // Split only: lanes in the group (all share the widths and neighbourhoods)
+    WQ_HD_INLINE int lanes() const { return G; }
+    WQ_HD_INLINE bool writes(int k) const { return !Split || (k == 0 ? fsel == 0 : fsel < nf); }
+    WQ_HD_INLINE int count() const { return Split ? (fsel > 0 && fsel < nf ? 2 : 1) : nf; }
     WQ_HD_INLINE int freq(int k) const { return Split ? (k == 0 ? 0 : fsel) : k; }
 };
 
@@ -1648,7 +1649,7 @@ WQ_HD void body_build(int64_t c, int fsel, int G, double* lds, const DEnv& E, co
         }
         for (int k = 0; k < kSlots; ++k) {
             if (k >= fs.count()) break;
-            if (Split && k == 0 && fsel != 0) continue;  // frequency 0 is written by its own lane
+            if (!fs.writes(k)) continue;  // each frequency is written by its own lane
             const int f = fs.freq(k);
             Rc.ray_int[slot * F.nf + f] = inten[k];
             Rc.ray_phase[slot * F.nf + f] = A.phase[((int64_t)S.curr * F.nf + f) * F.N + i];
