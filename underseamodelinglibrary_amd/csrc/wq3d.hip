@@ -46,6 +46,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kBuildLanes = 32;  // lanes per eigenray candidate in k_build
+constexpr int kDetectBlocks = 16384;  // k_detect grid size to aim for (work units)
 
 #ifndef WQ3D_EMULATE
 __global__ void __launch_bounds__(kBlock) k_init(DEnv E, FanConst F, Arrays A, Slots S) {
@@ -329,6 +330,31 @@ void launch_detect(hipStream_t st, unsigned grid, unsigned groups, const FanCons
     hipLaunchKernelGGL(k_detect, dim3(grid, groups), dim3(kBlock), 0, st, F, A, S, R, per_group);
 #else
     const TargetBox* boxes = reinterpret_cast<const TargetBox*>(F.tgt_box);
+    if (std::getenv("WQ3D_EMU_TRACE")) {
+        // share of (8 x 8 ray tile, target block) pairs a wavefront scans
+        const int64_t td = (F.nde + 7) / 8, ta = (F.naz + 7) / 8;
+        int64_t scanned = 0, rays_ok = 0, rays_all = 0;
+        for (int64_t tile = 0; tile < td * ta; ++tile) {
+            std::vector<DetectWindow> Ws;
+            for (int l = 0; l < 64; ++l) {
+                const int64_t de = (tile / ta) * 8 + l / 8, az = (tile % ta) * 8 + l % 8;
+                DetectWindow W;
+                if (de < F.nde && az < F.naz && detect_ray_window(F, A, S, de * F.naz + az, W)) Ws.push_back(W);
+            }
+            for (int64_t b = 0; b < F.nblk; ++b) {
+                bool any = false;
+                for (const auto& W : Ws) {
+                    const bool ok = block_window_ok(W, boxes[b]);
+                    rays_ok += ok;
+                    any = any || ok;
+                }
+                rays_all += (int64_t)Ws.size();
+                scanned += any;
+            }
+        }
+        std::fprintf(stderr, "[emu] tile-block pairs scanned %.3f, ray-block pairs kept %.3f\n",
+                     (double)scanned / (double)(td * ta * F.nblk), (double)rays_ok / (double)std::max<int64_t>(1, rays_all));
+    }
     for (int64_t i = 0; i < F.N; ++i) {
         DetectWindow W;
         if (!detect_ray_window(F, A, S, i, W)) continue;
@@ -1084,7 +1110,7 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
     const unsigned dgrid = (unsigned)((tiles + 3) / 4);
     int64_t groups = 1;
     if (F.nt > 0) {
-        groups = std::max<int64_t>(1, (int64_t)4096 / std::max<unsigned>(1u, dgrid));
+        groups = std::max<int64_t>(1, (int64_t)kDetectBlocks / std::max<unsigned>(1u, dgrid));
         groups = std::min<int64_t>(groups, std::max<int64_t>(1, F.nblk / 4));
     }
     const int64_t per_group = F.nblk > 0 ? (F.nblk + groups - 1) / groups : 0;
