@@ -81,7 +81,7 @@ def main():
     q.step(args.warmup)
     q.sync()
     q.drain_eigenrays()
-    lib.check(lib.wq3d_set_timing(q._fan, 1))
+    lib.check(lib.wq3d_set_timing(q._fan, 2))  # HIP events around k_detect only
     lib.check(lib.wq3d_kernel_times(q._fan, (C.c_double * 8)(), (C.c_int64 * 8)(), 8))
     q.sync()
     t0 = time.perf_counter()

@@ -57,12 +57,20 @@ __global__ void __launch_bounds__(kBlock) k_init(DEnv E, FanConst F, Arrays A, S
 __global__ void __launch_bounds__(kBlock) k_reflect(DEnv E, FanConst F, Arrays A, Slots S,
                                                    Records Rc, double time, int64_t step_key) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // the eigenray search counters of this step (read by later launches)
+    if (i == 0) {
+        *Rc.cand_count = 0ull;
+        *Rc.hit_count = 0ull;
+    }
     if (i < F.N) body_reflect(i, E, F, A, S, Rc, time, step_key);
 }
 
 __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A, Slots S) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < F.N) body_advance(i, E, F, A, S);
+    if (i < F.N) {
+        body_advance(i, E, F, A, S);
+        store_window(F, A, S, i);
+    }
 }
 
 // Eigenray search, first stage: one thread per interior ray over this
@@ -104,7 +112,7 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
     const int64_t tile = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     const int64_t i = tile_ray(F, tile, lane);
     DetectWindow W;
-    const bool active = detect_ray_window(F, A, S, i, W);
+    const bool active = load_window(F, A, i, W);
     if (!__any(active)) return;  // wave-uniform: no barriers below
     const int64_t b_begin = (int64_t)blockIdx.y * blk_per_group;
     const int64_t b_end = (F.nblk < b_begin + blk_per_group) ? F.nblk : b_begin + blk_per_group;
@@ -311,6 +319,8 @@ void launch_reflect(hipStream_t st, unsigned grid, const DEnv& E, const FanConst
 #ifndef WQ3D_EMULATE
     hipLaunchKernelGGL(k_reflect, dim3(grid), dim3(kBlock), 0, st, E, F, A, S, R, t, key);
 #else
+    *R.cand_count = 0ull;
+    *R.hit_count = 0ull;
     for (int64_t i = 0; i < F.N; ++i) body_reflect(i, E, F, A, S, R, t, key);
 #endif
 }
@@ -320,7 +330,10 @@ void launch_advance(hipStream_t st, unsigned grid, const DEnv& E, const FanConst
 #ifndef WQ3D_EMULATE
     hipLaunchKernelGGL(k_advance, dim3(grid), dim3(kBlock), 0, st, E, F, A, S);
 #else
-    for (int64_t i = 0; i < F.N; ++i) body_advance(i, E, F, A, S);
+    for (int64_t i = 0; i < F.N; ++i) {
+        body_advance(i, E, F, A, S);
+        store_window(F, A, S, i);
+    }
 #endif
 }
 
@@ -339,7 +352,7 @@ void launch_detect(hipStream_t st, unsigned grid, unsigned groups, const FanCons
             for (int l = 0; l < 64; ++l) {
                 const int64_t de = (tile / ta) * 8 + l / 8, az = (tile % ta) * 8 + l % 8;
                 DetectWindow W;
-                if (de < F.nde && az < F.naz && detect_ray_window(F, A, S, de * F.naz + az, W)) Ws.push_back(W);
+                if (de < F.nde && az < F.naz && load_window(F, A, de * F.naz + az, W)) Ws.push_back(W);
             }
             for (int64_t b = 0; b < F.nblk; ++b) {
                 bool any = false;
@@ -357,7 +370,7 @@ void launch_detect(hipStream_t st, unsigned grid, unsigned groups, const FanCons
     }
     for (int64_t i = 0; i < F.N; ++i) {
         DetectWindow W;
-        if (!detect_ray_window(F, A, S, i, W)) continue;
+        if (!load_window(F, A, i, W)) continue;
         for (int64_t b = 0; b < F.nblk; ++b) {
             if (!block_window_ok(W, boxes[b])) continue;
             const int64_t t_end = std::min<int64_t>(F.nt, (b + 1) * kTargetBlock);
@@ -523,7 +536,7 @@ struct wq3d_fan {
     int32_t* error = nullptr;
     int64_t pending_steps = 0;  // steps since the last device->host record move
     // timing
-    bool timing = false;
+    int timing = 0;  // 0 off, 1 every kernel, 2 k_detect only
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
     std::vector<hipEvent_t> event_pool;
     double kernel_ms[8] = {0};
@@ -668,14 +681,15 @@ enum { KID_INIT = 0, KID_REFLECT = 1, KID_ADVANCE = 2, KID_DETECT = 3, KID_BUILD
 template <class Launch>
 hipError_t timed(wq3d_fan* f, int kid, Launch&& launch) {
     hipEvent_t a = nullptr, b = nullptr;
-    if (f->timing) {
+    const bool on = f->timing == 1 || (f->timing == 2 && kid == KID_DETECT);
+    if (on) {
         a = take_event(f);
         b = take_event(f);
         (void)rt::event_record(a, f->stream);
     }
     launch();
     hipError_t e = rt::last_error();
-    if (f->timing) {
+    if (on) {
         (void)rt::event_record(b, f->stream);
         f->events.push_back({kid, {a, b}});
     }
@@ -1039,7 +1053,8 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         int32_t** ifields[] = {&A.srf, &A.btm, &A.cst, &A.upr, &A.lwr};
         for (int32_t** p : ifields)
             if (f->mem.alloc(p, n4) != hipSuccess) return bail(fail(WQ3D_ERR_DEVICE, "state allocation failed"));
-        if (f->mem.alloc(&A.edge, n4) != hipSuccess || f->mem.alloc(&A.flag, (size_t)N) != hipSuccess)
+        if (f->mem.alloc(&A.edge, n4) != hipSuccess || f->mem.alloc(&A.flag, (size_t)N) != hipSuccess ||
+            f->mem.alloc(&A.win, (size_t)N * 16) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "state allocation failed"));
     }
     // record buffers
@@ -1140,8 +1155,6 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
         }
         if (F.nt > 0 && F.has_spreading) {
             const Slots S = slots_of(f);
-            // candidate and hit counters are adjacent
-            HIP_TRY(rt::memset_async(f->R.cand_count, 0, 2 * sizeof(unsigned long long), f->stream));
             HIP_TRY(timed(f, KID_DETECT, [&] {
                 launch_detect(f->stream, dgrid, (unsigned)groups, F, f->A, S, f->R, per_group);
             }));
@@ -1339,7 +1352,8 @@ wq3d_status wq3d_search_stats(wq3d_fan* f, int64_t* window_hits, int64_t* candid
 
 wq3d_status wq3d_set_timing(wq3d_fan* f, int32_t enable) {
     if (!f) return fail(WQ3D_ERR_ARGUMENT, "null fan");
-    f->timing = enable != 0;
+    if (enable < 0 || enable > 2) return fail(WQ3D_ERR_ARGUMENT, "timing mode must be 0, 1 or 2");
+    f->timing = enable;
     return WQ3D_OK;
 }
 

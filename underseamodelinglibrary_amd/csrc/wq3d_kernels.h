@@ -50,6 +50,7 @@ struct Arrays {
     int32_t *srf, *btm, *cst, *upr, *lwr;
     uint8_t* edge;
     uint8_t* flag;              // [N] reflection kind for the coming step
+    double* win;                // [N][16] eigenray pre-test window of the step (store_window)
 };
 
 struct FanConst {
@@ -671,18 +672,15 @@ WQ_HD void body_advance(int64_t i, const DEnv& E, const FanConst& F, const Array
     A.flag[i] = reflect_kind(E, F, y3.pr, y3.pt, y3.pp, y2.nr, y2.nt, y2.np);
 }
 
-// --------------------------------------------------------------- detect
-// Time-CPA pre-test of detect_eigenrays (wave_queue.cc:372-395): thread per
-// interior ray, targets staged through LDS in tiles; each pair that passes is
-// appended to the candidate list with one atomic per wavefront.
-
 // ---------------------------------------------------------------- detect
-// The eigenray search of detect_eigenrays (wave_queue.cc:372-398) is split in
-// two: k_detect evaluates, for every (target, interior ray), the time-CPA
-// pre-test and an exact subset of is_closest_ray (the four in-plane
-// neighbours at the current time, with its skip and tie rules); survivors
-// are re-tested in full by k_build.  Dropping a pair here is only ever done
-// on a condition for which is_closest_ray would return false.
+// The eigenray search of detect_eigenrays (wave_queue.cc:372-398) is split
+// in three: k_detect applies a window test equivalent to the time-CPA
+// pre-test's outcome (see detect_window) to every (target, interior ray)
+// pair; k_exact decides the pairs inside the window with the exact pre-test
+// and the current-time part of is_closest_ray for the four in-plane
+// neighbours; k_build runs is_closest_ray in full on the survivors.  A pair
+// is only ever dropped on a condition for which is_closest_ray would return
+// false.
 //
 // Distance: the expression of wave_front.cc:227-232 with per-ray and
 // per-target terms hoisted.  It is the same IEEE value: 0.5*(a-b) equals
@@ -872,6 +870,37 @@ WQ_HD_INLINE bool detect_ray_window(const FanConst& F, const Arrays& A, const Sl
     const DPoint nxt = dpoint(A, at(S.next, N, i));
     const DPoint prv = dpoint(A, at(S.prev, N, i));
     W = detect_window(F, lin_diff(nxt, cur), lin_diff(prv, cur));
+    return true;
+}
+
+// The window of ray i for the current step, computed once by k_advance
+// (after the new next slot's update()) and read by every target group of
+// k_detect: {form.k, form.a[6], w, prev.k, prev.a[6], active}.
+WQ_HD_INLINE void store_window(const FanConst& F, const Arrays& A, const Slots& S, int64_t i) {
+    double* out = A.win + i * 16;
+    DetectWindow W;
+    if (F.nt <= 0 || !detect_ray_window(F, A, S, i, W)) {
+        out[15] = 0.0;
+        return;
+    }
+    out[0] = W.form.k;
+    for (int j = 0; j < 6; ++j) out[1 + j] = W.form.a[j];
+    out[7] = W.w;
+    out[8] = W.prev.k;
+    for (int j = 0; j < 6; ++j) out[9 + j] = W.prev.a[j];
+    out[15] = 1.0;
+}
+
+WQ_HD_INLINE bool load_window(const FanConst& F, const Arrays& A, int64_t i, DetectWindow& W) {
+    if (i < 0 || i >= F.N) return false;
+    const double* in = A.win + i * 16;
+    if (in[15] == 0.0) return false;
+    W.form.k = in[0];
+    for (int j = 0; j < 6; ++j) W.form.a[j] = in[1 + j];
+    W.w = in[7];
+    W.prev.k = in[8];
+    for (int j = 0; j < 6; ++j) W.prev.a[j] = in[9 + j];
+    W.m = F.lin_margin;
     return true;
 }
 
