@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A
 // LDS and written to the global hit list a wavefront at a time (one atomic
 // per wavefront and flush).  Only the window form stays live, which keeps
 // the register count, and so the occupancy, where the scan needs it.
-constexpr int kQueue = 16;
+constexpr int kQueue = 24;
 
 struct alignas(16) LinFeatures {
     double f[6];
@@ -93,14 +93,18 @@ typedef const LinFeatures ConstLinFeatures;
 typedef const TargetBox ConstTargetBox;
 #endif
 
-// Rays of a wavefront form an 8 x 8 tile of the fan (DE x AZ), so their
-// pre-test windows cross the target field in nearby places and whole target
-// blocks can be skipped (block_window_ok) for all 64 lanes at once.
+// Rays of a wavefront form a tile of the fan, 8 AZ wide and 8 * kRaysPerLane
+// DE deep, lane l holding the rays (8 r + l / 8, l % 8) of the tile for
+// r < kRaysPerLane: each target's features, loaded once per wavefront, serve
+// kRaysPerLane rays per lane.  Nearby rays' pre-test windows cross the
+// target field in nearby places, so whole target blocks can be skipped
+// (block_window_ok) for the whole wavefront at once.
 constexpr int kTile = 8;
+constexpr int kRaysPerLane = 2;
 
-__device__ __forceinline__ int64_t tile_ray(const FanConst& F, int64_t tile, int lane) {
+__device__ __forceinline__ int64_t tile_ray(const FanConst& F, int64_t tile, int lane, int r) {
     const int64_t tiles_az = (F.naz + kTile - 1) / kTile;
-    const int64_t de = (tile / tiles_az) * kTile + lane / kTile;
+    const int64_t de = (tile / tiles_az) * (kTile * kRaysPerLane) + r * kTile + lane / kTile;
     const int64_t az = (tile % tiles_az) * kTile + lane % kTile;
     return (de < F.nde && az < F.naz) ? de * F.naz + az : -1;
 }
@@ -110,14 +114,21 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
     __shared__ uint32_t s_q[kQueue][kBlock];
     const int lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    const int64_t i = tile_ray(F, tile, lane);
-    DetectWindow W;
-    const bool active = load_window(F, A, i, W);
-    if (!__any(active)) return;  // wave-uniform: no barriers below
+    int64_t ray[kRaysPerLane];
+    DetectWindow W[kRaysPerLane];
+    bool active[kRaysPerLane];
+    bool any_active = false;
+    for (int r = 0; r < kRaysPerLane; ++r) {
+        ray[r] = tile_ray(F, tile, lane, r);
+        active[r] = load_window(F, A, ray[r], W[r]);
+        any_active = any_active || active[r];
+    }
+    if (!__any(any_active)) return;  // wave-uniform: no barriers below
     const int64_t b_begin = (int64_t)blockIdx.y * blk_per_group;
     const int64_t b_end = (F.nblk < b_begin + blk_per_group) ? F.nblk : b_begin + blk_per_group;
     int qn = 0;
-    // write this wavefront's queued hits (scan positions) to the global list
+    // write this wavefront's queued hits (scan position, ray of the lane) to
+    // the global list
     auto flush = [&]() {
         int incl = qn;
         for (int off = 1; off < 64; off <<= 1) {
@@ -132,8 +143,10 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
         const unsigned long long mine = base + (unsigned long long)(incl - qn);
         for (int j = 0; j < qn; ++j) {
             const unsigned long long at_ = mine + (unsigned long long)j;
+            const uint32_t e = s_q[j][threadIdx.x];
+            const int64_t i = ray[e >> 31];
             if ((int64_t)at_ < Rc.hit_cap)
-                Rc.hit[at_] = ((unsigned long long)s_q[j][threadIdx.x] << 32) | (unsigned long long)i;
+                Rc.hit[at_] = ((unsigned long long)(e & 0x7fffffffu) << 32) | (unsigned long long)i;
             else
                 atomicOr(Rc.error, 2);
         }
@@ -143,16 +156,23 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
     const ConstTargetBox* boxes = (const ConstTargetBox*)(F.tgt_box);
     for (int64_t b = b_begin; b < b_end; ++b) {
         const TargetBox box = boxes[b];
-        const bool need = active && block_window_ok(W, box);
-        if (!__any(need)) continue;
+        bool need[kRaysPerLane];
+        bool any_need = false;
+        for (int r = 0; r < kRaysPerLane; ++r) {
+            need[r] = active[r] && block_window_ok(W[r], box);
+            any_need = any_need || need[r];
+        }
+        if (!__any(any_need)) continue;
         auto visit = [&](const LinFeatures& fc, int64_t t) {
-            if (need && detect_window_ok(W, fc.f)) {
-                s_q[qn][threadIdx.x] = (uint32_t)t;
-                ++qn;
+            for (int r = 0; r < kRaysPerLane; ++r) {
+                if (need[r] && detect_window_ok(W[r], fc.f)) {
+                    s_q[qn][threadIdx.x] = (uint32_t)t | ((uint32_t)r << 31);
+                    ++qn;
+                }
             }
         };
         // groups of four targets: one batch of scalar loads, at most four
-        // queue entries (the queues are flushed while they have room)
+        // queue entries per ray (the queues are flushed while they have room)
         const int64_t t_end = (F.nt < (b + 1) * kTargetBlock) ? F.nt : (b + 1) * kTargetBlock;
         int64_t t = b * kTargetBlock;
         for (; t + 4 <= t_end; t += 4) {
@@ -161,11 +181,11 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
             visit(f1, t + 1);
             visit(f2, t + 2);
             visit(f3, t + 3);
-            if (__any(qn > kQueue - 4)) flush();
+            if (__any(qn > kQueue - 4 * kRaysPerLane)) flush();
         }
         for (; t < t_end; ++t) {
             visit(cf[t], t);
-            if (__any(qn == kQueue)) flush();
+            if (__any(qn > kQueue - kRaysPerLane)) flush();
         }
     }
     flush();
@@ -1121,7 +1141,7 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
     const DEnv& E = f->env->dev;
     // the detect grid: 4 ray tiles (wavefronts) per block in x, target
     // blocks split into groups in y so that the grid has enough blocks
-    const int64_t tiles = ((F.nde + 7) / 8) * ((F.naz + 7) / 8);
+    const int64_t tiles = ((F.nde + kTile * kRaysPerLane - 1) / (kTile * kRaysPerLane)) * ((F.naz + kTile - 1) / kTile);
     const unsigned dgrid = (unsigned)((tiles + 3) / 4);
     int64_t groups = 1;
     if (F.nt > 0) {
