@@ -101,6 +101,10 @@ typedef const TargetBox ConstTargetBox;
 // (block_window_ok) for the whole wavefront at once.
 constexpr int kTile = 8;
 constexpr int kRaysPerLane = 2;
+// a queue entry holds the scan position below bit 30 and the lane's ray
+// (0 .. kRaysPerLane-1) above it
+constexpr int kQueueRayShift = 30;
+static_assert(kRaysPerLane <= 4, "queue entries hold two bits of ray");
 
 __device__ __forceinline__ int64_t tile_ray(const FanConst& F, int64_t tile, int lane, int r) {
     const int64_t tiles_az = (F.naz + kTile - 1) / kTile;
@@ -144,9 +148,9 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
         for (int j = 0; j < qn; ++j) {
             const unsigned long long at_ = mine + (unsigned long long)j;
             const uint32_t e = s_q[j][threadIdx.x];
-            const int64_t i = ray[e >> 31];
+            const int64_t i = ray[e >> kQueueRayShift];
             if ((int64_t)at_ < Rc.hit_cap)
-                Rc.hit[at_] = ((unsigned long long)(e & 0x7fffffffu) << 32) | (unsigned long long)i;
+                Rc.hit[at_] = ((unsigned long long)(e & ((1u << kQueueRayShift) - 1u)) << 32) | (unsigned long long)i;
             else
                 atomicOr(Rc.error, 2);
         }
@@ -166,7 +170,7 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
         auto visit = [&](const LinFeatures& fc, int64_t t) {
             for (int r = 0; r < kRaysPerLane; ++r) {
                 if (need[r] && detect_window_ok(W[r], fc.f)) {
-                    s_q[qn][threadIdx.x] = (uint32_t)t | ((uint32_t)r << 31);
+                    s_q[qn][threadIdx.x] = (uint32_t)t | ((uint32_t)r << kQueueRayShift);
                     ++qn;
                 }
             }
@@ -852,7 +856,7 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
     const int64_t nt = std::max<int64_t>(0, d->num_t1) * std::max<int64_t>(0, d->num_t2);
     if (nt > 0 && (!d->target_rho || !d->target_theta || !d->target_phi))
         return fail(WQ3D_ERR_ARGUMENT, "target arrays missing");
-    if (nt > 0x7fffffffLL) return fail(WQ3D_ERR_UNSUPPORTED, "too many targets");
+    if (nt > (1LL << 30)) return fail(WQ3D_ERR_UNSUPPORTED, "more than 2^30 targets");
     const bool has_spreading = d->de.size >= 3 && d->az.size >= 3;
     if (has_spreading && d->spreading != WQ3D_SPREADING_HYBRID_GAUSSIAN)
         return fail(WQ3D_ERR_UNSUPPORTED, "only HYBRID_GAUSSIAN spreading is supported");
