@@ -47,6 +47,13 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kBuildLanes = 32;  // lanes per eigenray candidate in k_build
 constexpr int kDetectBlocks = 16384;  // k_detect grid size to aim for (work units)
+// k_detect ray tiles: 8 AZ x (8 * kRaysPerLane) DE per wavefront
+constexpr int kTile = 8;
+constexpr int kRaysPerLane = 2;
+// a k_detect queue entry holds the scan position below bit 30 and the lane's
+// ray (0 .. kRaysPerLane-1) above it
+constexpr int kQueueRayShift = 30;
+static_assert(kRaysPerLane <= 4, "queue entries hold two bits of ray");
 
 #ifndef WQ3D_EMULATE
 __global__ void __launch_bounds__(kBlock) k_init(DEnv E, FanConst F, Arrays A, Slots S) {
@@ -99,12 +106,6 @@ typedef const TargetBox ConstTargetBox;
 // kRaysPerLane rays per lane.  Nearby rays' pre-test windows cross the
 // target field in nearby places, so whole target blocks can be skipped
 // (block_window_ok) for the whole wavefront at once.
-constexpr int kTile = 8;
-constexpr int kRaysPerLane = 2;
-// a queue entry holds the scan position below bit 30 and the lane's ray
-// (0 .. kRaysPerLane-1) above it
-constexpr int kQueueRayShift = 30;
-static_assert(kRaysPerLane <= 4, "queue entries hold two bits of ray");
 
 __device__ __forceinline__ int64_t tile_ray(const FanConst& F, int64_t tile, int lane, int r) {
     const int64_t tiles_az = (F.naz + kTile - 1) / kTile;
