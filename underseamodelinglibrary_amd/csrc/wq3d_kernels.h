@@ -1001,7 +1001,8 @@ WQ_HD_INLINE int64_t cube_ray(const FanConst& F, int de, int az, int nde, int na
 
 // wave_queue::is_closest_ray (wave_queue.cc:411-488) on the distances D of
 // the whole (time, DE, AZ) neighbourhood, checked in the reference's order.
-WQ_HD bool is_closest_ray(const BuildCtx& b, int64_t t, int de, int az, const double D[3][3][3]) {
+WQ_HD_INLINE bool is_closest_ray(const BuildCtx& b, int64_t t, int de, int az, const double D[3][3][3],
+                          const double edge_cur[3][3]) {
     const FanConst& F = b.F;
     const int max_az = F.naz - 1;
     const bool branch = F.tgt_branch[t] != 0;
@@ -1013,7 +1014,7 @@ WQ_HD bool is_closest_ray(const BuildCtx& b, int64_t t, int de, int az, const do
             const int a = (int)(i % F.naz);
             if (a == 0 && !F.az_boundary) continue;
             if (a == max_az) continue;
-            const bool edge = b.A.edge[at(b.S.curr, F.N, i)] != 0;
+            const bool edge = edge_cur[nde][naz] != 0.0;  // edge of ray i in the current slot
             if (branch) {
                 if (edge) continue;
             } else if (nde != 1) {
@@ -1040,7 +1041,7 @@ WQ_HD bool is_closest_ray(const BuildCtx& b, int64_t t, int de, int az, const do
 }
 
 // wave_queue::make_taylor_coeff (wave_queue.cc:929-977)
-WQ_HD void make_taylor_coeff(const double v[3][3][3], const double delta[3], double& center,
+WQ_HD_INLINE void make_taylor_coeff(const double v[3][3][3], const double delta[3], double& center,
                                   double g[3], double H[3][3]) {
     const double d0 = 2.0 * delta[0];
     const double d1 = 2.0 * delta[1];
@@ -1090,7 +1091,7 @@ WQ_HD_INLINE double norm3(const double g[3]) {
 }
 
 // collision_location of another ray (used by compute_offsets)
-WQ_HD void collision_at(const BuildCtx& b, int64_t i, double tw, double pos[3]) {
+WQ_HD_INLINE void collision_at(const BuildCtx& b, int64_t i, double tw, double pos[3]) {
     const Arrays& A = b.A;
     const int64_t N = b.F.N;
     const int64_t kp = at(b.S.prev, N, i), kc = at(b.S.curr, N, i), kn = at(b.S.next, N, i);
@@ -1106,34 +1107,23 @@ WQ_HD void collision_at(const BuildCtx& b, int64_t i, double tw, double pos[3]) 
 }
 
 // wave_queue::compute_offsets (wave_queue.cc:708-924)
-WQ_HD void compute_offsets(const BuildCtx& b, int64_t t, int de, int az, const double D[3][3][3],
-                                const double delta[3], double offset[3], double distance[3]) {
+// The first stability test of compute_offsets at one neighbourhood point:
+// ray j in slot s (0 prev, 1 curr, 2 next) against the centre's counts.
+WQ_HD_INLINE bool counters_differ_at(const BuildCtx& b, int s, int64_t j, int s0, int b0, int c0) {
+    const int slot = s == 0 ? b.S.prev : (s == 1 ? b.S.curr : b.S.next);
+    const int64_t k = at(slot, b.F.N, j);
+    return b.A.srf[k] != s0 || b.A.btm[k] != b0 || b.A.cst[k] != c0 || (s == 0 && b.A.edge[k] != 0);
+}
+
+// counters_differ: some ray of the neighbourhood, in some of the three
+// slots, has surface/bottom/caustic counts other than the centre's, or is on
+// an edge in the prev slot (the first stability test, wave_queue.cc:720-745;
+// see counters_differ_at)
+WQ_HD_INLINE void compute_offsets(const BuildCtx& b, int64_t t, int de, int az, const double D[3][3][3],
+                                const double delta[3], bool counters_differ, double offset[3],
+                                double distance[3]) {
     const FanConst& F = b.F;
-    const Arrays& A = b.A;
-    const int64_t N = F.N;
-    bool unstable = false;
-    const int64_t ic = at(b.S.curr, N, (int64_t)de * F.naz + az);
-    const int s0 = A.srf[ic], b0 = A.btm[ic], c0 = A.cst[ic];
-    for (int nde = 0; nde < 3 && !unstable; ++nde) {
-        const int d = de + nde - 1;
-        for (int naz = 0; naz < 3 && !unstable; ++naz) {
-            int a = az + naz - 1;
-            if (F.az_boundary) {
-                if (az + naz == 0) {
-                    a = F.naz - 2;
-                } else if (a >= F.naz - 1) {
-                    a = 0;
-                }
-            }
-            const int64_t i = (int64_t)d * F.naz + a;
-            const int64_t kp = at(b.S.prev, N, i), kc = at(b.S.curr, N, i), kn = at(b.S.next, N, i);
-            if (A.srf[kp] != s0 || A.srf[kc] != s0 || A.srf[kn] != s0 || A.btm[kp] != b0 ||
-                A.btm[kc] != b0 || A.btm[kn] != b0 || A.cst[kp] != c0 || A.cst[kc] != c0 ||
-                A.cst[kn] != c0 || A.edge[kp]) {
-                unstable = true;
-            }
-        }
-    }
+    bool unstable = counters_differ;
     double center, g[3], H[3][3], inv[3][3];
     make_taylor_coeff(D, delta, center, g, H);
     double determinant = smax(1e-6, H[0][0] * (H[1][1] * H[2][2] - H[1][2] * H[2][1]) +
@@ -1268,7 +1258,30 @@ struct Spread {
     double spread[FreqSel<Split>::kSlots];
     double I_de[FreqSel<Split>::kSlots];
     double I_az[FreqSel<Split>::kSlots];
-    uint32_t dup[kMaxAz / 32];
+#if defined(WQ3D_EMULATE) && !defined(__HIP_DEVICE_COMPILE__)
+    uint32_t dup[kMaxAz / 32];  // the plain set of visited AZ cells, checks AzMarks
+#endif
+};
+
+// AZ cells an intensity_az walk has visited (the reference's duplicate test,
+// spreading_hybrid_gaussian.cc:264-425).  The walk visits az, then az-1,
+// az-2, ... and afterwards az+1, az+2, ... (indices modulo max_az), so the
+// visited cells are always two cyclic runs, az down to az-(dn-1) and az+1
+// up to az+up, and a test is two modular compares instead of a bitset.
+struct AzMarks {
+    int az, max_az, dn = 0, up = 0;
+    WQ_HD_INLINE int cyc(int x) const { return ((x % max_az) + max_az) % max_az; }
+    // true if cell a was visited; marks it otherwise (it is the next cell of
+    // the down run while up == 0, of the up run after)
+    WQ_HD_INLINE bool test_set(int a) {
+        if (cyc(az - a) < dn || cyc(a - az - 1) < up) return true;
+        if (up == 0 && cyc(az - a) == dn) {
+            ++dn;
+        } else {
+            ++up;
+        }
+        return false;
+    }
 };
 
 WQ_HD_INLINE double ray_dist(const BuildCtx& b, int slot, int d1, int a1, int d2,
@@ -1279,7 +1292,7 @@ WQ_HD_INLINE double ray_dist(const BuildCtx& b, int slot, int d1, int a1, int d2
 }
 
 // width_de (spreading_hybrid_gaussian.cc:430-495)
-WQ_HD double width_de(const BuildCtx& b, int de, int az, const double offset[3]) {
+WQ_HD_INLINE double width_de(const BuildCtx& b, int de, int az, const double offset[3]) {
     const FanConst& F = b.F;
     const double u = fabs(offset[0]) / F.dt;
     const double v = fabs(offset[2]) / seq_inc(F.az, az);
@@ -1306,7 +1319,7 @@ WQ_HD double width_de(const BuildCtx& b, int de, int az, const double offset[3])
 }
 
 // width_az (spreading_hybrid_gaussian.cc:500-571)
-WQ_HD double width_az(const BuildCtx& b, int de, int az, const double offset[3]) {
+WQ_HD_INLINE double width_az(const BuildCtx& b, int de, int az, const double offset[3]) {
     const FanConst& F = b.F;
     const double u = fabs(offset[0]) / F.dt;
     const double v = fabs(offset[1]) / seq_inc(F.de, de);
@@ -1334,33 +1347,46 @@ WQ_HD double width_az(const BuildCtx& b, int de, int az, const double offset[3])
     return 0.5 * ((1.0 - u) * length1 + u * length2);
 }
 
-// Widths of the cells one spreading walk visits, cell k of the walk being
-// cell(k).  A split build computes them L at a time (L = the lanes taking
-// part), lane g taking cell base + g, and hands each out by a shuffle; the
-// widths do not depend on the walk's state, so computing a few past the
-// walk's end changes nothing.  A plain build computes each width when it is
-// asked for.
-template <bool Split, class WidthOf>
-struct WidthStream {
-    WidthOf width_of;  // width of cell k of the walk; any k >= 0 must be safe
+// What a spreading walk reads of one cell: its width, the two norms it may
+// use and one flag (a virtual ray in DE, an edge in AZ).
+struct CellData {
+    double w, n0, n1;
+    bool flag;
+};
+
+// The cells one spreading walk visits, cell k of the walk being cell(k).  A
+// split build fetches them L at a time (L = the lanes of the group), lane g
+// fetching cell base + g, and hands each out by shuffles, so the walk's
+// sequential loop waits on no memory; the data do not depend on the walk's
+// state, so fetching a few cells past the walk's end changes nothing.  A
+// plain build fetches each cell when it is asked for.
+template <bool Split, class CellOf>
+struct CellStream {
+    CellOf cell_of;  // data of cell k of the walk; any k >= 0 must be safe
     int G, L, g;
     int base = 0;
-    double w = 0.0;
+    CellData c{0.0, 0.0, 0.0, false};
     bool loaded = false;
-    WQ_HD_INLINE double get(int k) {
-        if (!Split) return width_of(k);
+    WQ_HD_INLINE CellData get(int k) {
+        if (!Split) return cell_of(k);
         if (!loaded || k >= base + L) {
             base = k;
-            w = width_of(k + g);
+            c = cell_of(k + g);
             loaded = true;
         }
-        return group_shfl(w, k - base, G);
+        const int src = k - base;
+        CellData out;
+        out.w = group_shfl(c.w, src, G);
+        out.n0 = group_shfl(c.n0, src, G);
+        out.n1 = group_shfl(c.n1, src, G);
+        out.flag = group_shfl(c.flag ? 1.0 : 0.0, src, G) != 0.0;
+        return out;
     }
 };
 
-template <bool Split, class WidthOf>
-WQ_HD_INLINE WidthStream<Split, WidthOf> width_stream(const FreqSel<Split>& fs, WidthOf fn) {
-    return WidthStream<Split, WidthOf>{fn, fs.G, fs.lanes(), fs.fsel};
+template <bool Split, class CellOf>
+WQ_HD_INLINE CellStream<Split, CellOf> cell_stream(const FreqSel<Split>& fs, CellOf fn) {
+    return CellStream<Split, CellOf>{fn, fs.G, fs.lanes(), fs.fsel};
 }
 
 // Orders the LDS writes of a wavefront's lanes before their reads by other
@@ -1378,12 +1404,13 @@ WQ_HD_INLINE void wave_sync() {
 // time (lane g takes k = g, g + L, ...) and every lane reads all 27 back;
 // otherwise one thread computes them all.  fn(k, out, stride) writes out[0]
 // and, for two values per point, out[stride].
-template <bool Split, class PointFn>
-WQ_HD_INLINE void fill_cube(const FreqSel<Split>& fs, double* lds, int nval, double* out, PointFn fn) {
+template <int NVal, bool Split, class PointFn>
+WQ_HD_INLINE void fill_cube(const FreqSel<Split>& fs, double* lds, double* out, PointFn fn) {
     if (Split && lds != nullptr) {
         for (int k = fs.fsel; k < 27; k += fs.lanes()) fn(k, lds + k, 27);
         wave_sync();
-        for (int k = 0; k < 27 * nval; ++k) out[k] = lds[k];
+#pragma unroll
+        for (int k = 0; k < 27 * NVal; ++k) out[k] = lds[k];
         wave_sync();
     } else {
         for (int k = 0; k < 27; ++k) fn(k, out + k, 27);
@@ -1408,7 +1435,7 @@ WQ_HD_INLINE bool edge_at(const BuildCtx& b, int de, int az) {
 
 // intensity_de (spreading_hybrid_gaussian.cc:124-258)
 template <bool Split>
-WQ_HD void intensity_de(Spread<Split>& s, int de, int az, const double offset[3], const double distance[3]) {
+WQ_HD_INLINE void intensity_de(Spread<Split>& s, int de, int az, const double offset[3], const double distance[3]) {
     const BuildCtx& b = s.b;
     const double THRESHOLD = 1.002305238;
     const double* norm_de = b.F.norm_de;
@@ -1423,19 +1450,25 @@ WQ_HD void intensity_de(Spread<Split>& s, int de, int az, const double offset[3]
     cell_dist = L + cell_width;
     gaussian_add(s, cell_dist, cell_width, norm_de[d], s.I_de, false);
     if (s.I_de[0] < 1e-10) return;
-    auto down = width_stream(s.fs, [&](int k) { return de - 2 - k >= 0 ? width_de(b, de - 2 - k, az, offset) : 0.0; });
+    // cells de-2, de-3, ... 0: width, norm_de[d], norm_de[d+1], virtual ray
+    auto down = cell_stream(s.fs, [&](int k) {
+        const int dd = de - 2 - k;
+        if (dd < 0) return CellData{0.0, 0.0, 0.0, false};
+        return CellData{width_de(b, dd, az, offset), norm_de[dd], norm_de[dd + 1],
+                        edge_at(b, dd + 1, az) && edge_at(b, dd, az)};
+    });
     for (d = de - 2; d >= 0; --d) {
-        const bool virtual_ray = edge_at(b, d + 1, az) && edge_at(b, d, az);
+        const CellData cd = down.get(de - 2 - d);
+        const bool virtual_ray = cd.flag;
         double nn;
         cell_dist += cell_width;
-        const double wd = down.get(de - 2 - d);
         if (virtual_ray) {
             cell_dist += cell_width;
-            nn = norm_de[d + 1];
+            nn = cd.n1;
         } else {
-            cell_width = wd;
+            cell_width = cd.w;
             cell_dist += cell_width;
-            nn = norm_de[d];
+            nn = cd.n0;
         }
         const double old_tl = s.I_de[0];
         gaussian_add(s, cell_dist, cell_width, nn, s.I_de, false);
@@ -1445,19 +1478,25 @@ WQ_HD void intensity_de(Spread<Split>& s, int de, int az, const double offset[3]
     cell_width = initial_width;
     cell_dist = L - cell_width;
     const int size = b.F.nde - 1;
-    auto up = width_stream(s.fs, [&](int k) { return de + 1 + k < size ? width_de(b, de + 1 + k, az, offset) : 0.0; });
+    // cells de+1, de+2, ... size-1: width, norm_de[d], norm_de[d-1], virtual ray
+    auto up = cell_stream(s.fs, [&](int k) {
+        const int dd = de + 1 + k;
+        if (dd >= size) return CellData{0.0, 0.0, 0.0, false};
+        return CellData{width_de(b, dd, az, offset), norm_de[dd], norm_de[dd - 1],
+                        edge_at(b, dd + 1, az) && edge_at(b, dd, az)};
+    });
     for (d = de + 1; d < size; ++d) {
-        const bool virtual_ray = edge_at(b, d + 1, az) && edge_at(b, d, az);
+        const CellData cd = up.get(d - de - 1);
+        const bool virtual_ray = cd.flag;
         double nn;
         cell_dist -= cell_width;
-        const double wd = up.get(d - de - 1);
         if (virtual_ray) {
             cell_dist -= cell_width;
-            nn = norm_de[d - 1];
+            nn = cd.n1;
         } else {
-            cell_width = wd;
+            cell_width = cd.w;
             cell_dist -= cell_width;
-            nn = norm_de[d];
+            nn = cd.n0;
         }
         const double old_tl = s.I_de[0];
         gaussian_add(s, cell_dist, cell_width, nn, s.I_de, false);
@@ -1467,16 +1506,20 @@ WQ_HD void intensity_de(Spread<Split>& s, int de, int az, const double offset[3]
 }
 
 template <bool Split>
-WQ_HD_INLINE bool dup_test_set(Spread<Split>& s, int a) {
+WQ_HD_INLINE bool dup_test_set(Spread<Split>& s, AzMarks& m, int a) {
+    const bool was = m.test_set(a);
+#if defined(WQ3D_EMULATE) && !defined(__HIP_DEVICE_COMPILE__)
     const uint32_t bit = 1u << (a & 31);
-    const bool was = (s.dup[a >> 5] & bit) != 0;
+    const bool set_was = (s.dup[a >> 5] & bit) != 0;
     s.dup[a >> 5] |= bit;
+    wq3d_emulation_check(set_was == was, "AZ visited-cell runs", (double)a);
+#endif
     return was;
 }
 
 // intensity_az (spreading_hybrid_gaussian.cc:264-425)
 template <bool Split>
-WQ_HD void intensity_az(Spread<Split>& s, int de, int az, const double offset[3], const double distance[3]) {
+WQ_HD_INLINE void intensity_az(Spread<Split>& s, int de, int az, const double offset[3], const double distance[3]) {
     const BuildCtx& b = s.b;
     const FanConst& F = b.F;
     const double THRESHOLD = 1.002305238;
@@ -1490,19 +1533,22 @@ WQ_HD void intensity_az(Spread<Split>& s, int de, int az, const double offset[3]
         az_lower = 0;
         az_upper = max_az - 1;
     }
+#if defined(WQ3D_EMULATE) && !defined(__HIP_DEVICE_COMPILE__)
     for (int w = 0; w < (naz + 31) / 32; ++w) s.dup[w] = 0u;
+#endif
+    AzMarks marks{az, max_az};
     auto norm = [&](int dd, int aa) {
         return (dd >= max_de) ? F.norm_az[1 * (int64_t)naz + aa] : F.norm_az[(int64_t)dd * naz + aa];
     };
     int a = az;
-    dup_test_set(s, a);
+    dup_test_set(s, marks, a);
     double cell_width = width_az(b, de, a, offset);
     const double initial_width = cell_width;
     const double L = distance[2];
     double cell_dist = L - cell_width;
     gaussian_add(s, cell_dist, cell_width, norm(de, a), s.I_az, true);
     a = (az < 1) ? max_az - 1 : az - 1;
-    dup_test_set(s, a);
+    dup_test_set(s, marks, a);
     cell_width = width_az(b, de, a, offset);
     cell_dist = L + cell_width;
     gaussian_add(s, cell_dist, cell_width, norm(de, a), s.I_az, false);
@@ -1512,21 +1558,24 @@ WQ_HD void intensity_az(Spread<Split>& s, int de, int az, const double offset[3]
     } else {
         --a;
     }
-    // the walk visits a, a-1, ... wrapping from 0 to max_az-1
+    // the walk visits a, a-1, ... wrapping from 0 to max_az-1: width, norm,
+    // edge of each
     const int a_down = a;
-    auto down = width_stream(s.fs, [&](int k) {
+    auto down = cell_stream(s.fs, [&](int k) {
         const int c = ((a_down - k) % max_az + max_az) % max_az;
-        return width_az(b, de, c, offset);
+        const double n = norm(de, c);
+        return CellData{width_az(b, de, c, offset), n, n, edge_at(b, de, c)};
     });
     int kd = 0;
     while ((a % max_az) != az_lower) {
-        if (dup_test_set(s, a)) break;
-        if (edge_at(b, de, a)) break;
+        const CellData cd = down.get(kd++);
+        if (dup_test_set(s, marks, a)) break;
+        if (cd.flag) break;
         cell_dist += cell_width;
-        cell_width = down.get(kd++);
+        cell_width = cd.w;
         cell_dist += cell_width;
         const double old_tl = s.I_az[0];
-        gaussian_add(s, cell_dist, cell_width, norm(de, a), s.I_az, false);
+        gaussian_add(s, cell_dist, cell_width, cd.n0, s.I_az, false);
         if (s.I_az[0] / old_tl < THRESHOLD) break;
         if (a == 0) {
             a += max_az - 1;
@@ -1537,18 +1586,23 @@ WQ_HD void intensity_az(Spread<Split>& s, int de, int az, const double offset[3]
     cell_width = initial_width;
     cell_dist = L - cell_width;
     a = az + 1;
-    // the walk visits az+1, az+2, ... modulo max_az
-    auto up = width_stream(s.fs, [&](int k) { return width_az(b, de, (az + 1 + k) % max_az, offset); });
+    // the walk visits az+1, az+2, ... modulo max_az: width, norm, edge
+    auto up = cell_stream(s.fs, [&](int k) {
+        const int c = (az + 1 + k) % max_az;
+        const double n = norm(de, c);
+        return CellData{width_az(b, de, c, offset), n, n, edge_at(b, de, c)};
+    });
     int ku = 0;
     while ((a % max_az) != az_upper) {
         if (a == max_az) a = 0;
-        if (dup_test_set(s, a)) break;
-        if (edge_at(b, de, a)) break;
+        const CellData cd = up.get(ku++);
+        if (dup_test_set(s, marks, a)) break;
+        if (cd.flag) break;
         cell_dist -= cell_width;
-        cell_width = up.get(ku++);
+        cell_width = cd.w;
         cell_dist -= cell_width;
         const double old_tl = s.I_az[0];
-        gaussian_add(s, cell_dist, cell_width, norm(de, a), s.I_az, false);
+        gaussian_add(s, cell_dist, cell_width, cd.n0, s.I_az, false);
         if (s.I_az[0] / old_tl < THRESHOLD) break;
         ++a;
     }
@@ -1566,7 +1620,7 @@ WQ_HD_INLINE void ndir_angles(const Arrays& A, int64_t k, double& de, double& az
 // every lane runs the same control flow (the walks stop on frequency 0,
 // which every lane evaluates), so the group stays converged.
 template <bool Split>
-WQ_HD void body_build(int64_t c, int fsel, int G, double* lds, const DEnv& E, const FanConst& F,
+WQ_HD_INLINE void body_build(int64_t c, int fsel, int G, double* lds, const DEnv& E, const FanConst& F,
                       const Arrays& A, const Slots& S, const Records& Rc, double time, int64_t step_key) {
     const BuildCtx b{E, F, A, S};
     const FreqSel<Split> fs{F.nf, fsel, G};
@@ -1577,17 +1631,29 @@ WQ_HD void body_build(int64_t c, int fsel, int G, double* lds, const DEnv& E, co
         const int64_t i = (int64_t)(key & 0xffffffffull);
         const int de = (int)(i / F.naz), az = (int)(i % F.naz);
         const int sl[3] = {S.prev, S.curr, S.next};
-        double D[3][3][3];
-        fill_cube(fs, lds, 1, &D[0][0][0], [&](int k, double* out, int) {
-            out[0] = dist2_slot(b, sl[k / 9], cube_ray(F, de, az, (k / 3) % 3, k % 3), t);
+        // distances of the neighbourhood, and the edge flags of its rays in
+        // the current slot (the second value of the s = 1 points)
+        // (and, for compute_offsets, whether any point's counts differ from
+        // the centre's)
+        const int64_t ic0 = at(S.curr, F.N, i);
+        const int s0 = A.srf[ic0], b0 = A.btm[ic0], c0 = A.cst[ic0];
+        bool differ = false;
+        double DE[2][3][3][3];
+        fill_cube<2>(fs, lds, &DE[0][0][0][0], [&](int k, double* out, int stride) {
+            const int64_t j = cube_ray(F, de, az, (k / 3) % 3, k % 3);
+            out[0] = dist2_slot(b, sl[k / 9], j, t);
+            out[stride] = (k / 9 == 1 && A.edge[at(S.curr, F.N, j)] != 0) ? 1.0 : 0.0;
+            differ = differ || counters_differ_at(b, k / 9, j, s0, b0, c0);
         });
-        if (!is_closest_ray(b, t, de, az, D)) return;
+        if (Split) differ = group_any(differ, G);
+        const double(&D)[3][3][3] = DE[0];
+        if (!is_closest_ray(b, t, de, az, D, DE[1][1])) return;
         // build_eigenray (wave_queue.cc:495-702)
         const int64_t ic = at(S.curr, F.N, i);
         if (above_bounce(F, A, ic)) return;
         const double delta[3] = {F.dt, seq_inc(F.de, de), seq_inc(F.az, az)};
         double offset[3], distance[3];
-        compute_offsets(b, t, de, az, D, delta, offset, distance);
+        compute_offsets(b, t, de, az, D, delta, differ, offset, distance);
         Spread<Split> sp{b, fs};
         double ts_c, ts_g[3];
         sound_speed(E.profile, F.tgt_r[t], F.tgt_t[t], F.tgt_p[t], F.R, ts_c, ts_g, false);
@@ -1639,7 +1705,7 @@ WQ_HD void body_build(int64_t c, int fsel, int G, double* lds, const DEnv& E, co
         if (Split) above = group_any(above, G);
         if (!above) return;
         double VW[2][3][3][3];  // launch-angle DE and AZ of the neighbourhood's directions
-        fill_cube(fs, lds, 2, &VW[0][0][0][0], [&](int k, double* out, int stride) {
+        fill_cube<2>(fs, lds, &VW[0][0][0][0], [&](int k, double* out, int stride) {
             ndir_angles(A, at(sl[k / 9], F.N, cube_ray(F, de, az, (k / 3) % 3, k % 3)), out[0], out[stride]);
         });
         const double(&V)[3][3][3] = VW[0];
