@@ -289,6 +289,12 @@ wq3d_status wq3d_kernel_times(wq3d_fan* fan, double* ms_per_launch, int64_t* lau
  * passed the exact pre-test).  Synchronises the fan's stream. */
 wq3d_status wq3d_search_stats(wq3d_fan* fan, int64_t* window_hits, int64_t* candidates);
 
+/* Diagnostic with no counterpart in the reference: the (target, ray) pairs
+ * put to the eigenray search's window test (pairs in target blocks the
+ * whole-block test did not rule out) since the fan was created or since the
+ * last call, which resets the count.  Synchronises the fan's stream. */
+wq3d_status wq3d_search_totals(wq3d_fan* fan, int64_t* pairs_tested);
+
 #ifdef __cplusplus
 }
 #endif

@@ -130,3 +130,23 @@ def test_detect_prefilter_matches_exact_comparisons():
     pairs, mismatches, window, passes, blocks = (int(x) for x in out.split("counts")[-1].split())
     assert mismatches == 0
     assert pairs > 100000 and window > passes > 100 and blocks > 1000
+
+
+def test_search_totals_count_and_reset():
+    """wq3d_search_totals: the pairs put to the window test over a run lie
+    between the last step's window hits and rays x targets x steps, and a
+    second call reads zero."""
+    out = _run("""
+        import ctypes as C
+        sc = scenarios.config1(n_targets=(4, 4), time_max=3.0)
+        g = wave_queue(*sc.args(), lib=lib)
+        g._lib.check(g._lib.wq3d_step(g._fan, sc.steps))
+        h, c, t, t2 = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        lib.check(lib.wq3d_search_stats(g._fan, C.byref(h), C.byref(c)))
+        lib.check(lib.wq3d_search_totals(g._fan, C.byref(t)))
+        lib.check(lib.wq3d_search_totals(g._fan, C.byref(t2)))
+        print("counts", h.value, c.value, t.value, t2.value, sc.num_rays() * sc.num_targets() * sc.steps)
+    """)
+    hits, cands, tested, again, bound = (int(x) for x in out.split("counts")[-1].split())
+    assert cands <= hits <= tested and 0 < tested < bound
+    assert again == 0

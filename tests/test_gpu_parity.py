@@ -5,6 +5,8 @@ travel time within 1e-9 relative, intensity within 0.01 dB, angles within
 1e-6 deg; wavefront state within 1e-10 scaled; eigenverb / reflection
 records with identical keys and values within 1e-9.
 """
+import textwrap
+
 import numpy as np
 import pytest
 
@@ -168,3 +170,48 @@ def test_listener_replay_order(oracle_mod):
         o.step()
     assert lg.calls == lo.calls
     assert sum(1 for c in lg.calls if c[0] == "ray") == 3
+
+
+def test_search_counts_match_emulation():
+    """The eigenray search on the GPU puts the same (target, ray) pairs to
+    the window test, and finds the same window hits and candidates, as the
+    serial CPU build of the same kernels: the per-wavefront block culling
+    and hit queues drop and duplicate nothing."""
+    import emulation
+    sc_src = """
+        from underseamodelinglibrary_amd import scenarios
+        from underseamodelinglibrary_amd.usml.types import seq_linear
+        sc = scenarios.config2_analytic(n_targets=(20, 20), time_max=5.0)
+        sc.de = seq_linear(-80.0, 1.0, 80.0)
+        sc.az = seq_linear(0.0, 6.0, 354.0)
+    """
+    script = textwrap.dedent(sc_src) + textwrap.dedent("""
+        import ctypes as C
+        import emulation
+        from underseamodelinglibrary_amd.usml.waveq3d import wave_queue
+        lib = emulation.load()
+        q = wave_queue(*sc.args(), lib=lib)
+        lib.check(lib.wq3d_step(q._fan, sc.steps))
+        h, c, t = C.c_int64(), C.c_int64(), C.c_int64()
+        lib.check(lib.wq3d_search_stats(q._fan, C.byref(h), C.byref(c)))
+        lib.check(lib.wq3d_search_totals(q._fan, C.byref(t)))
+        print("counts", h.value, c.value, t.value)
+    """)
+    res = emulation.run_script_under_asan(script, timeout=600)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-4000:]
+    want = [int(x) for x in res.stdout.split("counts")[-1].split()]
+    ns = {}
+    exec(textwrap.dedent(sc_src), ns)
+    sc = ns["sc"]
+    import ctypes as C
+    print(f"[parity] search counts: {sc.num_rays()} rays, {sc.num_targets()} targets, "
+          f"{sc.steps} steps", flush=True)
+    g = wave_queue(*sc.args())
+    g._lib.check(g._lib.wq3d_step(g._fan, sc.steps))
+    h, c, t = C.c_int64(), C.c_int64(), C.c_int64()
+    g._lib.check(g._lib.wq3d_search_stats(g._fan, C.byref(h), C.byref(c)))
+    g._lib.check(g._lib.wq3d_search_totals(g._fan, C.byref(t)))
+    assert [h.value, c.value, t.value] == want
+    assert 0 < h.value and t.value <= sc.num_rays() * sc.num_targets() * sc.steps
+    g._lib.check(g._lib.wq3d_search_totals(g._fan, C.byref(t)))
+    assert t.value == 0

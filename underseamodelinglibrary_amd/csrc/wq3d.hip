@@ -46,14 +46,15 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kBuildLanes = 32;  // lanes per eigenray candidate in k_build
-constexpr int kDetectBlocks = 16384;  // k_detect grid size to aim for (work units)
-// k_detect ray tiles: 8 AZ x (8 * kRaysPerLane) DE per wavefront
-constexpr int kTile = 8;
-constexpr int kRaysPerLane = 2;
-// a k_detect queue entry holds the scan position below bit 30 and the lane's
-// ray (0 .. kRaysPerLane-1) above it
-constexpr int kQueueRayShift = 30;
-static_assert(kRaysPerLane <= 4, "queue entries hold two bits of ray");
+// k_detect: ray tiles of kTile x kTile (AZ x DE, the tiles of the window
+// records), one ray per lane, and hit keys buffered per wavefront before a
+// flush
+constexpr int kTile = kWinTile;
+constexpr int kHitQueue = 128;
+// MI355X accelerator dies (XCDs), each with its own L2; consecutive
+// workgroups of a launch are dispatched to them in turn
+constexpr int kXcds = 8;
+static_assert(kTargetBlock == 64, "k_detect tests one target per lane");
 
 #ifndef WQ3D_EMULATE
 __global__ void __launch_bounds__(kBlock) k_init(DEnv E, FanConst F, Arrays A, Slots S) {
@@ -80,120 +81,151 @@ __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A
     }
 }
 
-// Eigenray search, first stage: one thread per interior ray over this
-// block's target group, one linear form and one compare per (ray, target)
-// pair (detect_window_ok), with the target's features wave-uniform in SGPRs
-// (scalar loads).  The few pairs inside the window are queued per lane in
-// LDS and written to the global hit list a wavefront at a time (one atomic
-// per wavefront and flush).  Only the window form stays live, which keeps
-// the register count, and so the occupancy, where the scan needs it.
-constexpr int kQueue = 24;
-
 struct alignas(16) LinFeatures {
     double f[6];
 };
 #if defined(__HIP_DEVICE_COMPILE__)
-typedef __attribute__((address_space(4))) const LinFeatures ConstLinFeatures;  // constant address space
-typedef __attribute__((address_space(4))) const TargetBox ConstTargetBox;
+// constant address space: wave-uniform reads become scalar loads
+typedef __attribute__((address_space(4))) const LinFeatures ConstLinFeatures;
 #else
 typedef const LinFeatures ConstLinFeatures;
-typedef const TargetBox ConstTargetBox;
 #endif
 
-// Rays of a wavefront form a tile of the fan, 8 AZ wide and 8 * kRaysPerLane
-// DE deep, lane l holding the rays (8 r + l / 8, l % 8) of the tile for
-// r < kRaysPerLane: each target's features, loaded once per wavefront, serve
-// kRaysPerLane rays per lane.  Nearby rays' pre-test windows cross the
-// target field in nearby places, so whole target blocks can be skipped
-// (block_window_ok) for the whole wavefront at once.
-
-__device__ __forceinline__ int64_t tile_ray(const FanConst& F, int64_t tile, int lane, int r) {
-    const int64_t tiles_az = (F.naz + kTile - 1) / kTile;
-    const int64_t de = (tile / tiles_az) * (kTile * kRaysPerLane) + r * kTile + lane / kTile;
-    const int64_t az = (tile % tiles_az) * kTile + lane % kTile;
-    return (de < F.nde && az < F.naz) ? de * F.naz + az : -1;
-}
-
-__global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S, Records Rc,
-                                                  int64_t blk_per_group) {
-    __shared__ uint32_t s_q[kQueue][kBlock];
+// Eigenray search, first stage.  A wavefront takes a tile of 8 x 8
+// neighbouring rays and one super-block of targets (kSuperBlock blocks of
+// 64, consecutive on the targets' Morton curve).  Each lane holds one ray's
+// candidate region (DetectCell: the pre-test window and the four neighbour
+// forms) and tests it against the super-block's box, then against each
+// block's box (block_cell_ok).  For the blocks some ray kept, a lane becomes
+// one of the block's 64 targets: the kept rays are visited one at a time,
+// their forms read from the ray's lane (readlane), and every lane tests its
+// target (detect_cell_ok, in stages so that later forms are read only while
+// some target is still in).  The few pairs that pass are buffered per
+// wavefront in LDS and written to the global hit list with one atomic per
+// flush.
+//
+// The grid is one-dimensional and XCD-aware: workgroups go round-robin to
+// the kXcds dies, so workgroup L runs on die L % kXcds.  Each die gets every
+// kXcds-th group of four tiles and runs all super-blocks of one such tile
+// group back to back, so the tiles' records are fetched into one die's L2
+// and reused there by every super-block.  Workgroups past the last tile
+// group (padding to a multiple of kXcds) return at once.
+__global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S, Records Rc, int64_t quads) {
+    __shared__ unsigned long long s_hit[kBlock / 64][kHitQueue];
     const int lane = threadIdx.x & 63;
-    const int64_t tile = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    int64_t ray[kRaysPerLane];
-    DetectWindow W[kRaysPerLane];
-    bool active[kRaysPerLane];
-    bool any_active = false;
-    for (int r = 0; r < kRaysPerLane; ++r) {
-        ray[r] = tile_ray(F, tile, lane, r);
-        active[r] = load_window(F, A, ray[r], W[r]);
-        any_active = any_active || active[r];
+    const int wv = threadIdx.x >> 6;
+    const int64_t L = blockIdx.x;
+    const int64_t k = L / kXcds;
+    const int64_t quad = L % kXcds + kXcds * (k / F.nsup);
+    const int64_t sup = k % F.nsup;
+    if (quad >= quads) return;  // whole workgroup, before the one barrier
+    // this lane's ray: tile of 8 AZ x 8 DE rays
+    int64_t ray = -1;
+    {
+        const int64_t tile = quad * (kBlock / 64) + wv;
+        const int64_t tiles_az = (F.naz + kTile - 1) / kTile;
+        const int64_t de = (tile / tiles_az) * kTile + lane / kTile;
+        const int64_t az = (tile % tiles_az) * kTile + lane % kTile;
+        if (de < F.nde && az < F.naz) ray = de * F.naz + az;
     }
-    if (!__any(any_active)) return;  // wave-uniform: no barriers below
-    const int64_t b_begin = (int64_t)blockIdx.y * blk_per_group;
-    const int64_t b_end = (F.nblk < b_begin + blk_per_group) ? F.nblk : b_begin + blk_per_group;
-    int qn = 0;
-    // write this wavefront's queued hits (scan position, ray of the lane) to
-    // the global list
+    // the super-block's box and its blocks' boxes, staged in LDS for all
+    // four wavefronts
+    __shared__ double s_box[kSuperBlock + 1][8];
+    __shared__ uint32_t s_br[kSuperBlock + 1];
+    const int64_t b_begin = sup * kSuperBlock;
+    const int nbk = (int)((F.nblk - b_begin < kSuperBlock) ? F.nblk - b_begin : kSuperBlock);
+    if (threadIdx.x < (kSuperBlock + 1) * 8) {
+        const int q = threadIdx.x >> 3, e = threadIdx.x & 7;
+        const int64_t bi = q < nbk ? b_begin + q : F.nblk + sup;  // slot kSuperBlock (and unused ones): the super box
+        s_box[q][e] = F.tgt_box[bi * 8 + e];
+        if (e == 0) s_br[q] = F.blk_branch[bi];
+    }
+    __syncthreads();
+    auto box_at = [&](int q) {
+        TargetBox B;
+        B.at0 = s_box[q][0];
+        B.hx = s_box[q][1];
+        B.bt0 = s_box[q][2];
+        B.hy = s_box[q][3];
+        B.r_lo = s_box[q][4];
+        B.r_hi = s_box[q][5];
+        B.s_lo = s_box[q][6];
+        B.s_hi = s_box[q][7];
+        return B;
+    };
+    DetectCell C;
+    bool active = load_cell(F, A, S, ray, C);
+    active = active && block_cell_ok(C, box_at(kSuperBlock), s_br[kSuperBlock] != 0);
+    if (!__any(active)) return;  // wave-uniform; no barriers below
+    // the blocks each ray keeps, decided for all blocks before any pair test
+    unsigned long long keep[kSuperBlock];
+    for (int q = 0; q < kSuperBlock; ++q)
+        keep[q] = __ballot(active && q < nbk && block_cell_ok(C, box_at(q), s_br[q] != 0));
+    unsigned qn = 0;                // keys in this wavefront's buffer
+    unsigned long long tested = 0;  // (target, ray) pairs put to the pair test
     auto flush = [&]() {
-        int incl = qn;
-        for (int off = 1; off < 64; off <<= 1) {
-            const int v = __shfl_up(incl, off);
-            if (lane >= off) incl += v;
-        }
-        const int total = __shfl(incl, 63);
-        if (total == 0) return;
+        wave_sync();
         unsigned long long base = 0;
-        if (lane == 63) base = atomicAdd(Rc.hit_count, (unsigned long long)total);
-        base = __shfl(base, 63);
-        const unsigned long long mine = base + (unsigned long long)(incl - qn);
-        for (int j = 0; j < qn; ++j) {
-            const unsigned long long at_ = mine + (unsigned long long)j;
-            const uint32_t e = s_q[j][threadIdx.x];
-            const int64_t i = ray[e >> kQueueRayShift];
+        if (lane == 0) base = atomicAdd(Rc.hit_count, (unsigned long long)qn);
+        base = __shfl(base, 0);
+        for (unsigned e = lane; e < qn; e += 64) {
+            const unsigned long long at_ = base + e;
             if ((int64_t)at_ < Rc.hit_cap)
-                Rc.hit[at_] = ((unsigned long long)(e & ((1u << kQueueRayShift) - 1u)) << 32) | (unsigned long long)i;
+                Rc.hit[at_] = s_hit[wv][e];
             else
                 atomicOr(Rc.error, 2);
         }
         qn = 0;
+        wave_sync();
+    };
+    // one target's pair test for every lane's ray; the later forms only
+    // while some lane is still in
+    auto visit = [&](bool in, const double* f, int64_t t, bool bbr) {
+        if (!__any(in)) return;
+        in = in && detect_window_prev_ok(C.W, f);
+        if (!__any(in)) return;
+        const bool tbr = bbr && F.scan_branch[t] != 0;
+        in = in && (tbr || detect_cell_nb_ok(C, f));
+        const unsigned long long hm = __ballot(in);
+        if (!hm) return;
+        const unsigned cnt = (unsigned)__popcll(hm);
+        if (qn + cnt > (unsigned)kHitQueue) flush();
+        if (in)
+            s_hit[wv][qn + (unsigned)__popcll(hm & ((1ull << lane) - 1ull))] =
+                ((unsigned long long)t << 32) | (unsigned long long)ray;
+        qn += cnt;
     };
     const ConstLinFeatures* cf = (const ConstLinFeatures*)(F.tgt_lin);
-    const ConstTargetBox* boxes = (const ConstTargetBox*)(F.tgt_box);
-    for (int64_t b = b_begin; b < b_end; ++b) {
-        const TargetBox box = boxes[b];
-        bool need[kRaysPerLane];
-        bool any_need = false;
-        for (int r = 0; r < kRaysPerLane; ++r) {
-            need[r] = active[r] && block_window_ok(W[r], box);
-            any_need = any_need || need[r];
-        }
-        if (!__any(any_need)) continue;
-        auto visit = [&](const LinFeatures& fc, int64_t t) {
-            for (int r = 0; r < kRaysPerLane; ++r) {
-                if (need[r] && detect_window_ok(W[r], fc.f)) {
-                    s_q[qn][threadIdx.x] = (uint32_t)t | ((uint32_t)r << kQueueRayShift);
-                    ++qn;
-                }
-            }
-        };
-        // groups of four targets: one batch of scalar loads, at most four
-        // queue entries per ray (the queues are flushed while they have room)
-        const int64_t t_end = (F.nt < (b + 1) * kTargetBlock) ? F.nt : (b + 1) * kTargetBlock;
-        int64_t t = b * kTargetBlock;
+    for (int q = 0; q < nbk; ++q) {
+        const unsigned long long kq = keep[q];
+        if (!kq) continue;
+        const bool need = (kq >> lane) & 1ull;
+        const int64_t t0 = (b_begin + q) * kTargetBlock;
+        const int64_t t_end = (F.nt < t0 + kTargetBlock) ? F.nt : t0 + kTargetBlock;
+        tested += (unsigned long long)__popcll(kq) * (unsigned long long)(t_end - t0);
+        const bool bbr = s_br[q] != 0;
+        // four targets per round: one batch of scalar loads, four window
+        // forms as independent chains
+        int64_t t = t0;
         for (; t + 4 <= t_end; t += 4) {
             const LinFeatures f0 = cf[t], f1 = cf[t + 1], f2 = cf[t + 2], f3 = cf[t + 3];
-            visit(f0, t);
-            visit(f1, t + 1);
-            visit(f2, t + 2);
-            visit(f3, t + 3);
-            if (__any(qn > kQueue - 4 * kRaysPerLane)) flush();
+            const bool i0 = need && detect_window_in(C.W, f0.f);
+            const bool i1 = need && detect_window_in(C.W, f1.f);
+            const bool i2 = need && detect_window_in(C.W, f2.f);
+            const bool i3 = need && detect_window_in(C.W, f3.f);
+            if (!__any(i0 || i1 || i2 || i3)) continue;
+            visit(i0, f0.f, t, bbr);
+            visit(i1, f1.f, t + 1, bbr);
+            visit(i2, f2.f, t + 2, bbr);
+            visit(i3, f3.f, t + 3, bbr);
         }
         for (; t < t_end; ++t) {
-            visit(cf[t], t);
-            if (__any(qn > kQueue - kRaysPerLane)) flush();
+            const LinFeatures fc = cf[t];
+            visit(need && detect_window_in(C.W, fc.f), fc.f, t, bbr);
         }
     }
-    flush();
+    if (qn) flush();
+    if (lane == 0 && tested) atomicAdd(&Rc.scan[blockIdx.x % kScanSlots], tested);
 }
 
 // Eigenray search, second stage: the exact pre-test (detect_pair) of every
@@ -362,45 +394,34 @@ void launch_advance(hipStream_t st, unsigned grid, const DEnv& E, const FanConst
 #endif
 }
 
-void launch_detect(hipStream_t st, unsigned grid, unsigned groups, const FanConst& F, const Arrays& A,
-                   const Slots& S, const Records& R, int64_t per_group) {
+void launch_detect(hipStream_t st, int64_t quads, const FanConst& F, const Arrays& A, const Slots& S,
+                   const Records& R) {
 #ifndef WQ3D_EMULATE
-    hipLaunchKernelGGL(k_detect, dim3(grid, groups), dim3(kBlock), 0, st, F, A, S, R, per_group);
+    const int64_t grid = kXcds * ((quads + kXcds - 1) / kXcds) * F.nsup;
+    hipLaunchKernelGGL(k_detect, dim3((unsigned)grid), dim3(kBlock), 0, st, F, A, S, R, quads);
 #else
     const TargetBox* boxes = reinterpret_cast<const TargetBox*>(F.tgt_box);
-    if (std::getenv("WQ3D_EMU_TRACE")) {
-        // share of (8 x 8 ray tile, target block) pairs a wavefront scans
-        const int64_t td = (F.nde + 7) / 8, ta = (F.naz + 7) / 8;
-        int64_t scanned = 0, rays_ok = 0, rays_all = 0;
-        for (int64_t tile = 0; tile < td * ta; ++tile) {
-            std::vector<DetectWindow> Ws;
-            for (int l = 0; l < 64; ++l) {
-                const int64_t de = (tile / ta) * 8 + l / 8, az = (tile % ta) * 8 + l % 8;
-                DetectWindow W;
-                if (de < F.nde && az < F.naz && load_window(F, A, de * F.naz + az, W)) Ws.push_back(W);
-            }
-            for (int64_t b = 0; b < F.nblk; ++b) {
-                bool any = false;
-                for (const auto& W : Ws) {
-                    const bool ok = block_window_ok(W, boxes[b]);
-                    rays_ok += ok;
-                    any = any || ok;
-                }
-                rays_all += (int64_t)Ws.size();
-                scanned += any;
-            }
-        }
-        std::fprintf(stderr, "[emu] tile-block pairs scanned %.3f, ray-block pairs kept %.3f\n",
-                     (double)scanned / (double)(td * ta * F.nblk), (double)rays_ok / (double)std::max<int64_t>(1, rays_all));
-    }
+    const bool trace = std::getenv("WQ3D_EMU_TRACE") != nullptr;
+    int64_t rays = 0, rays_kept = 0, ray_blocks = 0, window_blocks = 0, cell_blocks = 0;
     for (int64_t i = 0; i < F.N; ++i) {
-        DetectWindow W;
-        if (!load_window(F, A, i, W)) continue;
+        DetectCell C;
+        if (!load_cell(F, A, S, i, C)) continue;
+        rays += 1;
+        if (trace) {
+            for (int64_t b = 0; b < F.nblk; ++b) window_blocks += block_window_ok(C.W, boxes[b]);
+            ray_blocks += F.nblk;
+        }
+        bool kept = false;
         for (int64_t b = 0; b < F.nblk; ++b) {
-            if (!block_window_ok(W, boxes[b])) continue;
+            const int64_t u = b / kSuperBlock;
+            if (!block_cell_ok(C, boxes[F.nblk + u], F.blk_branch[F.nblk + u] != 0)) continue;
+            kept = true;
+            if (!block_cell_ok(C, boxes[b], F.blk_branch[b] != 0)) continue;
+            cell_blocks += 1;
             const int64_t t_end = std::min<int64_t>(F.nt, (b + 1) * kTargetBlock);
+            R.scan[0] += (unsigned long long)(t_end - b * kTargetBlock);
             for (int64_t t = b * kTargetBlock; t < t_end; ++t) {
-                if (!detect_window_ok(W, F.tgt_lin + t * 6)) continue;
+                if (!detect_cell_ok(C, F.tgt_lin + t * 6, F.scan_branch[t] != 0)) continue;
                 const unsigned long long off = (*R.hit_count)++;
                 if ((int64_t)off < R.hit_cap)
                     R.hit[off] = ((unsigned long long)t << 32) | (unsigned long long)i;
@@ -408,7 +429,13 @@ void launch_detect(hipStream_t st, unsigned grid, unsigned groups, const FanCons
                     *R.error |= 2;
             }
         }
+        rays_kept += kept;
     }
+    if (trace)
+        std::fprintf(stderr, "[emu] rays searched %lld, kept by a super-block box %lld; ray-block pairs kept: "
+                     "window %.4f, cell %.5f; hits %llu\n", (long long)rays, (long long)rays_kept,
+                     (double)window_blocks / (double)std::max<int64_t>(1, ray_blocks),
+                     (double)cell_blocks / (double)std::max<int64_t>(1, ray_blocks), *R.hit_count);
 #endif
 }
 
@@ -1002,18 +1029,30 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         }
         F.lin_margin = lin_margin(rmax);
         F.nblk = (nt + kTargetBlock - 1) / kTargetBlock;
-        std::vector<TargetBox> box((size_t)std::max<int64_t>(1, F.nblk));
-        for (int64_t b = 0; b < F.nblk; ++b) {
-            BoxAcc acc;
-            for (int64_t k = b * kTargetBlock; k < std::min<int64_t>(nt, (b + 1) * kTargetBlock); ++k) {
-                const uint32_t t = order[k];
-                acc.add(th[t], ph[t], d->target_rho[t], ts[t]);
+        F.nsup = (F.nblk + kSuperBlock - 1) / kSuperBlock;
+        std::vector<TargetBox> box((size_t)(F.nblk + F.nsup));
+        std::vector<uint8_t> blk_br((size_t)(F.nblk + F.nsup), 0), scan_br((size_t)nt);
+        for (int64_t u = 0; u < F.nsup; ++u) {
+            BoxAcc sup;
+            for (int64_t b = u * kSuperBlock; b < std::min<int64_t>(F.nblk, (u + 1) * kSuperBlock); ++b) {
+                BoxAcc acc;
+                for (int64_t k = b * kTargetBlock; k < std::min<int64_t>(nt, (b + 1) * kTargetBlock); ++k) {
+                    const uint32_t t = order[k];
+                    acc.add(th[t], ph[t], d->target_rho[t], ts[t]);
+                    sup.add(th[t], ph[t], d->target_rho[t], ts[t]);
+                    scan_br[k] = br[t];
+                    blk_br[b] |= br[t];
+                    blk_br[F.nblk + u] |= br[t];
+                }
+                box[b] = acc.box();
             }
-            box[b] = acc.box();
+            box[F.nblk + u] = sup.box();
         }
         if (f->mem.upload(&F.tgt_lin, lin.data(), lin.size()) != hipSuccess ||
             f->mem.upload(&F.tgt_perm, order.data(), order.size()) != hipSuccess ||
             f->mem.upload(&F.tgt_box, reinterpret_cast<const double*>(box.data()), box.size() * 8) != hipSuccess ||
+            f->mem.upload(&F.blk_branch, blk_br.data(), blk_br.size()) != hipSuccess ||
+            f->mem.upload(&F.scan_branch, scan_br.data(), scan_br.size()) != hipSuccess ||
             f->mem.upload(&F.tgt_th, th.data(), nt) != hipSuccess ||
             f->mem.upload(&F.tgt_ph, ph.data(), nt) != hipSuccess ||
             f->mem.upload(&F.tgt_r2, r2.data(), nt) != hipSuccess ||
@@ -1079,7 +1118,7 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         for (int32_t** p : ifields)
             if (f->mem.alloc(p, n4) != hipSuccess) return bail(fail(WQ3D_ERR_DEVICE, "state allocation failed"));
         if (f->mem.alloc(&A.edge, n4) != hipSuccess || f->mem.alloc(&A.flag, (size_t)N) != hipSuccess ||
-            f->mem.alloc(&A.win, (size_t)N * 16) != hipSuccess)
+            f->mem.alloc(&A.win, (size_t)(win_tiles(F) * kWinField * kWinStride)) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "state allocation failed"));
     }
     // record buffers
@@ -1097,15 +1136,16 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
             f->mem.alloc(&R.verb_power, R.verb_cap * F.nf) != hipSuccess ||
             f->mem.alloc(&R.refl, R.refl_cap) != hipSuccess ||
             f->mem.alloc(&R.cand, R.cand_cap) != hipSuccess || f->mem.alloc(&R.hit, R.hit_cap) != hipSuccess ||
-            f->mem.alloc(&f->counters, 5) != hipSuccess || f->mem.alloc(&f->error, 1) != hipSuccess)
+            f->mem.alloc(&f->counters, 5 + kScanSlots) != hipSuccess || f->mem.alloc(&f->error, 1) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "record allocation failed"));
         R.ray_count = f->counters + 0;
         R.verb_count = f->counters + 1;
         R.refl_count = f->counters + 2;
         R.cand_count = f->counters + 3;
         R.hit_count = f->counters + 4;
+        R.scan = f->counters + 5;
         R.error = f->error;
-        if (rt::memset(f->counters, 0, 5 * sizeof(unsigned long long)) != hipSuccess ||
+        if (rt::memset(f->counters, 0, (5 + kScanSlots) * sizeof(unsigned long long)) != hipSuccess ||
             rt::memset(f->error, 0, sizeof(int32_t)) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "memset failed"));
     }
@@ -1144,16 +1184,10 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
     const FanConst& F = f->F;
     const unsigned grid = (unsigned)((F.N + kBlock - 1) / kBlock);
     const DEnv& E = f->env->dev;
-    // the detect grid: 4 ray tiles (wavefronts) per block in x, target
-    // blocks split into groups in y so that the grid has enough blocks
-    const int64_t tiles = ((F.nde + kTile * kRaysPerLane - 1) / (kTile * kRaysPerLane)) * ((F.naz + kTile - 1) / kTile);
-    const unsigned dgrid = (unsigned)((tiles + 3) / 4);
-    int64_t groups = 1;
-    if (F.nt > 0) {
-        groups = std::max<int64_t>(1, (int64_t)kDetectBlocks / std::max<unsigned>(1u, dgrid));
-        groups = std::min<int64_t>(groups, std::max<int64_t>(1, F.nblk / 4));
-    }
-    const int64_t per_group = F.nblk > 0 ? (F.nblk + groups - 1) / groups : 0;
+    // the detect grid: 4 ray tiles (wavefronts) per workgroup times the
+    // super-blocks of targets
+    const int64_t tiles = ((F.nde + kTile - 1) / kTile) * ((F.naz + kTile - 1) / kTile);
+    const int64_t quads = (tiles + 3) / 4;
     for (int64_t s = 0; s < n; ++s) {
         // detect_reflections
         {
@@ -1181,7 +1215,7 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
         if (F.nt > 0 && F.has_spreading) {
             const Slots S = slots_of(f);
             HIP_TRY(timed(f, KID_DETECT, [&] {
-                launch_detect(f->stream, dgrid, (unsigned)groups, F, f->A, S, f->R, per_group);
+                launch_detect(f->stream, quads, F, f->A, S, f->R);
             }));
             HIP_TRY(timed(f, KID_EXACT, [&] { launch_exact(f->stream, F, f->A, S, f->R); }));
             const double t = f->time;
@@ -1375,6 +1409,18 @@ wq3d_status wq3d_search_stats(wq3d_fan* f, int64_t* window_hits, int64_t* candid
     return WQ3D_OK;
 }
 
+wq3d_status wq3d_search_totals(wq3d_fan* f, int64_t* pairs_tested) {
+    if (!f || !pairs_tested) return fail(WQ3D_ERR_ARGUMENT, "null argument");
+    HIP_TRY(rt::sync(f->stream));
+    *pairs_tested = 0;
+    if (!f->R.scan) return WQ3D_OK;
+    unsigned long long c[kScanSlots];
+    HIP_TRY(rt::memcpy(c, f->R.scan, sizeof(c), hipMemcpyDeviceToHost));
+    for (int k = 0; k < kScanSlots; ++k) *pairs_tested += (int64_t)c[k];
+    HIP_TRY(rt::memset_async(f->R.scan, 0, sizeof(c), f->stream));
+    return WQ3D_OK;
+}
+
 wq3d_status wq3d_set_timing(wq3d_fan* f, int32_t enable) {
     if (!f) return fail(WQ3D_ERR_ARGUMENT, "null fan");
     if (enable < 0 || enable > 2) return fail(WQ3D_ERR_ARGUMENT, "timing mode must be 0, 1 or 2");
@@ -1408,9 +1454,11 @@ wq3d_status wq3d_kernel_times(wq3d_fan* f, double* ms, int64_t* launches, int32_
 // the earth's radius, plus targets placed by bisection on the boundaries
 // where an exact comparison of the pre-test flips, decided once with the
 // linear-form pre-filter and once with the exact expressions alone.
-// Blocks of 8 consecutive generated targets check the block bound the same
-// way.  counts = {pairs, mismatches (pairs and rejected blocks), pairs
-// inside the window, passes, rejected blocks}.
+// The pair filter is the pre-test window alone and the window with the
+// neighbour forms (detect_cell_ok).  Blocks of 8 consecutive generated
+// targets check both block bounds the same way.  counts = {pairs,
+// mismatches (pairs and rejected blocks), pairs the filter keeps, passes,
+// blocks rejected by the cell bound}.
 static double emu_uniform(unsigned long long& s, double lo, double hi) {
     s = s * 6364136223846793005ull + 1442695040888963407ull;
     return lo + (hi - lo) * ((double)(s >> 11) * 0x1.0p-53);
@@ -1530,20 +1578,30 @@ int32_t wq3d_emu_check_detect(int64_t num_rays, int64_t num_targets, uint64_t se
                 F.lin_fmax[j] = std::max(F.lin_fmax[j], f6[j]);
             }
         }
-        const DetectWindow W = detect_window(F, lin_diff(fp.n, dr.cur), lin_diff(fp.p, dr.cur));
+        DetectCell cell;
+        cell.W = detect_window(F, lin_diff(fp.n, dr.cur), lin_diff(fp.p, dr.cur));
+        for (int n = 0; n < 4; ++n) cell.nb[n] = lin_diff(fp.nbp[n], dr.cur);
+        cell.use = dr.use;
         // blocks of 8 consecutive generated targets (the bisected ones sit
-        // together): a rejected block must hold no target inside the window
+        // together): a rejected block must hold no target the pair filter
+        // keeps; the bound is checked with and without the neighbour forms
         for (size_t b0 = 0; b0 < tg.size(); b0 += 8) {
             BoxAcc acc;
             const size_t b1 = std::min(tg.size(), b0 + 8);
-            for (size_t k = b0; k < b1; ++k) acc.add(0.5 * tg[k].t, 0.5 * tg[k].p, tg[k].r, wqm::sin(tg[k].t));
-            if (block_window_ok(W, acc.box())) continue;
-            counts[4] += 1;
+            bool br = false;
+            for (size_t k = b0; k < b1; ++k) {
+                acc.add(0.5 * tg[k].t, 0.5 * tg[k].p, tg[k].r, wqm::sin(tg[k].t));
+                br = br || tg[k].branch;
+            }
+            const bool keep_window = block_window_ok(cell.W, acc.box());
+            const bool keep_cell = block_cell_ok(cell, acc.box(), br);
+            counts[4] += keep_cell ? 0 : 1;
             for (size_t k = b0; k < b1; ++k) {
                 DTarget x;
                 double f6[6];
                 make(tg[k].r, tg[k].t, tg[k].p, x, f6);
-                counts[1] += detect_window_ok(W, f6) ? 1 : 0;
+                if (!keep_window) counts[1] += detect_window_ok(cell.W, f6) ? 1 : 0;
+                if (!keep_cell) counts[1] += detect_cell_ok(cell, f6, tg[k].branch) ? 1 : 0;
             }
         }
         for (const Tgt& g : tg) {
@@ -1551,11 +1609,13 @@ int32_t wq3d_emu_check_detect(int64_t num_rays, int64_t num_targets, uint64_t se
             double f6[6];
             make(g.r, g.t, g.p, x, f6);
             const bool ref = detect_pair(dr, fp, x, g.branch);
-            const bool window = detect_window_ok(W, f6);
-            const bool got = window && detect_pair(dr, fp, x, g.branch);
+            const bool window = detect_window_ok(cell.W, f6);
+            const bool kept = detect_cell_ok(cell, f6, g.branch);
             counts[0] += 1;
-            counts[1] += ref != got;
-            counts[2] += window;
+            counts[1] += ref != (window && detect_pair(dr, fp, x, g.branch));
+            counts[1] += ref != (kept && detect_pair(dr, fp, x, g.branch));
+            counts[1] += kept && !window;
+            counts[2] += kept;
             counts[3] += ref;
         }
     }

@@ -80,8 +80,12 @@ struct FanConst {
     // of each scan position, tgt_box one TargetBox per block.
     const double* tgt_lin;
     const uint32_t* tgt_perm;
+    // tgt_box: nblk block boxes, then nsup boxes of kSuperBlock consecutive
+    // blocks; blk_branch: the same boxes' flag "holds a source-branch target"
     const double* tgt_box;
-    int64_t nblk;
+    const uint8_t* blk_branch;
+    const uint8_t* scan_branch; // source-branch flag per scan position
+    int64_t nblk, nsup;
     double lin_margin;
     double lin_fmin[6], lin_fmax[6];  // feature ranges over all targets
     // spreading_hybrid_gaussian norms (spreading_hybrid_gaussian.cc:28-59)
@@ -117,11 +121,15 @@ struct Records {
     unsigned long long* cand;   // pairs that passed the exact pre-test
     unsigned long long* cand_count;
     int64_t cand_cap;
+    unsigned long long* scan;   // [kScanSlots] (target, ray) pairs put to the
+                                // window test, summed over steps
     int32_t* error;             // bit 0: record overflow, bit 1: candidate overflow,
                                 // bit 2: reflection chain exceeded its bound
 };
 
 constexpr int kMaxBounceChain = 64;
+// counters of the window test's pair count, spread to keep atomics apart
+constexpr int kScanSlots = 64;
 
 WQ_HD_INLINE int64_t at(int s, int64_t N, int64_t i) { return (int64_t)s * N + i; }
 
@@ -803,14 +811,20 @@ WQ_HD_INLINE DetectWindow detect_window(const FanConst& F, const LinForm& next, 
 // d2(prev) - d2(cur) is below -margin (prev surely closer than center).
 // The second form matters for rays whose window is wide, such as rays whose
 // three snapshots straddle a reflection.  NaN passes through.
-WQ_HD_INLINE bool detect_window_ok(const DetectWindow& W, const double f[6]) {
-    if (fabs(lin_eval(W.form, f)) > W.w) return false;
+WQ_HD_INLINE bool detect_window_in(const DetectWindow& W, const double f[6]) {
+    return !(fabs(lin_eval(W.form, f)) > W.w);
+}
+WQ_HD_INLINE bool detect_window_prev_ok(const DetectWindow& W, const double f[6]) {
     return !(lin_eval(W.prev, f) < -W.m);
+}
+WQ_HD_INLINE bool detect_window_ok(const DetectWindow& W, const double f[6]) {
+    return detect_window_in(W, f) && detect_window_prev_ok(W, f);
 }
 
 // Box of one block of targets, in the variables of the features: theta/2
 // and phi/2 as centre and half width, rho and sin(theta) as ranges.
 constexpr int kTargetBlock = 64;
+constexpr int kSuperBlock = 8;  // blocks per super-block (one k_detect work unit)
 struct TargetBox {
     double at0, hx, bt0, hy, r_lo, r_hi, s_lo, s_hi;
 };
@@ -821,8 +835,8 @@ struct TargetBox {
 // exactly; st and R are positive ranges.  false when no target of the box
 // can be inside the window (one more margin absorbs the rounding of this
 // bound and of the features).
-WQ_HD_INLINE bool block_window_ok(const DetectWindow& W, const TargetBox& B) {
-    const double* a = W.form.a;
+WQ_HD_INLINE void block_form_range(const LinForm& L, const TargetBox& B, double& lo, double& hi) {
+    const double* a = L.a;
     const double P0 = a[0] + a[1] * B.at0 + a[2] * (B.at0 * B.at0);
     const double P1 = fabs(a[1] + 2.0 * a[2] * B.at0) * B.hx;
     const double P2 = a[2] * (B.hx * B.hx);
@@ -836,10 +850,81 @@ WQ_HD_INLINE bool block_window_ok(const DetectWindow& W, const TargetBox& B) {
     const double sq_lo = fmin(fmin(B.s_lo * q_lo, B.s_hi * q_lo), fmin(B.s_lo * q_hi, B.s_hi * q_hi));
     const double sq_hi = fmax(fmax(B.s_lo * q_lo, B.s_hi * q_lo), fmax(B.s_lo * q_hi, B.s_hi * q_hi));
     const double g_lo = p_lo + sq_lo, g_hi = p_hi + sq_hi;
-    const double lo = W.form.k + fmin(B.r_lo * g_lo, B.r_hi * g_lo);
-    const double hi = W.form.k + fmax(B.r_lo * g_hi, B.r_hi * g_hi);
+    lo = L.k + fmin(B.r_lo * g_lo, B.r_hi * g_lo);
+    hi = L.k + fmax(B.r_lo * g_hi, B.r_hi * g_hi);
+}
+
+WQ_HD_INLINE bool block_window_ok(const DetectWindow& W, const TargetBox& B) {
+    double lo, hi;
+    block_form_range(W.form, B, lo, hi);
     const double w = W.w + W.m;
     return !(lo > w || hi < -w);
+}
+
+// In-plane neighbours of ray i that is_closest_ray compares a target
+// against (wave_queue.cc:423-478): DE-1, DE+1, AZ-1, AZ+1 with the AZ wrap
+// of a fan that closes on itself.  stat bit n: neighbour n is not excluded
+// by the AZ rule (the DE neighbours' edge flags are checked separately);
+// upper bit n: rejection on <= instead of <.
+WQ_HD_INLINE void ray_neighbours(const FanConst& F, int64_t i, int32_t nbj[4], uint32_t& stat, uint32_t& upper) {
+    const int de = (int)(i / F.naz), az = (int)(i % F.naz);
+    const int max_az = F.naz - 1;
+    const int dd[4] = {de - 1, de + 1, de, de};
+    const int nz[4] = {1, 1, 0, 2};
+    const int nd[4] = {0, 2, 1, 1};
+    stat = 0;
+    upper = 0;
+    for (int n = 0; n < 4; ++n) {
+        int a = az + nz[n] - 1;
+        if (F.az_boundary) {
+            if (az + nz[n] == 0) {
+                a = F.naz - 2;
+            } else if (a >= max_az) {
+                a = 0;
+            }
+        }
+        nbj[n] = (int32_t)((int64_t)dd[n] * F.naz + a);
+        stat |= (((a == 0 && !F.az_boundary) || a == max_az) ? 0u : 1u) << n;
+        upper |= ((nd[n] == 2 || nz[n] == 2) ? 1u : 0u) << n;
+    }
+}
+
+// The candidate region of one ray: the pre-test window and, for each
+// neighbour is_closest_ray compares against, d2(neighbour) - d2(cur) as a
+// linear form.  A target for which one of these is below -margin is closer
+// to that neighbour than to this ray (the rounding of the form is within
+// the margin), so is_closest_ray rejects it, unless the target has the
+// source-branch rule, which skips the neighbour comparison.
+struct DetectCell {
+    DetectWindow W;
+    LinForm nb[4];
+    uint32_t use;  // bit n: neighbour n is compared
+};
+
+WQ_HD_INLINE bool detect_cell_nb_ok(const DetectCell& C, const double f[6]) {
+    for (int n = 0; n < 4; ++n)
+        if (((C.use >> n) & 1u) && lin_eval(C.nb[n], f) < -C.W.m) return false;
+    return true;
+}
+
+WQ_HD_INLINE bool detect_cell_ok(const DetectCell& C, const double f[6], bool branch) {
+    return detect_window_ok(C.W, f) && (branch || detect_cell_nb_ok(C, f));
+}
+
+// Whole-block form of detect_cell_ok: false when no target of the box can
+// pass (blocks holding a source-branch target skip the neighbour part).
+// The neighbour bound gets one more margin for its own rounding, as the
+// window bound does.
+WQ_HD_INLINE bool block_cell_ok(const DetectCell& C, const TargetBox& B, bool branch) {
+    if (!block_window_ok(C.W, B)) return false;
+    if (branch) return true;
+    for (int n = 0; n < 4; ++n) {
+        if (!((C.use >> n) & 1u)) continue;
+        double lo, hi;
+        block_form_range(C.nb[n], B, lo, hi);
+        if (hi < -2.0 * C.W.m) return false;
+    }
+    return true;
 }
 
 // Ray points of the exact stage, read from the state arrays on demand.
@@ -873,34 +958,92 @@ WQ_HD_INLINE bool detect_ray_window(const FanConst& F, const Arrays& A, const Sl
     return true;
 }
 
-// The window of ray i for the current step, computed once by k_advance
-// (after the new next slot's update()) and read by every target group of
-// k_detect: {form.k, form.a[6], w, prev.k, prev.a[6], active}.
+// The candidate region of ray i for the current step, computed once by
+// k_advance (after the new next slot's update()) and read by every target
+// group of k_detect.  Record of kWinStride doubles: {form.k, form.a[6], w,
+// prev.k, prev.a[6], 1 + stat bits (0: ray not searched), nb[n].k,
+// nb[n].a[6] for n = 0..3}.  The neighbours' points are the curr slot's,
+// which k_advance does not write; their edge flags are written by the same
+// launch, so the reader applies them (cell_use).
+constexpr int kWinStride = 44;
+// Records are stored tile by tile (kWinTile x kWinTile rays, AZ fastest,
+// the k_detect wavefront's tile) and field-major within a tile, so that a
+// wavefront reads each field of its 64 rays as one contiguous 512-byte row.
+constexpr int kWinTile = 8;
+
+WQ_HD_INLINE int64_t win_tiles(const FanConst& F) {
+    return (int64_t)((F.nde + kWinTile - 1) / kWinTile) * ((F.naz + kWinTile - 1) / kWinTile);
+}
+
+// offset of field 0 of ray i's record; field q is at q * kWinTile^2 past it
+WQ_HD_INLINE int64_t win_slot(const FanConst& F, int64_t i) {
+    const int64_t de = i / F.naz, az = i % F.naz;
+    const int64_t tiles_az = (F.naz + kWinTile - 1) / kWinTile;
+    const int64_t tile = (de / kWinTile) * tiles_az + az / kWinTile;
+    return tile * (kWinTile * kWinTile) * kWinStride + (de % kWinTile) * kWinTile + az % kWinTile;
+}
+constexpr int kWinField = kWinTile * kWinTile;
+
 WQ_HD_INLINE void store_window(const FanConst& F, const Arrays& A, const Slots& S, int64_t i) {
-    double* out = A.win + i * 16;
+    double* out = A.win + win_slot(F, i);
     DetectWindow W;
     if (F.nt <= 0 || !detect_ray_window(F, A, S, i, W)) {
-        out[15] = 0.0;
+        out[15 * kWinField] = 0.0;
         return;
     }
     out[0] = W.form.k;
-    for (int j = 0; j < 6; ++j) out[1 + j] = W.form.a[j];
-    out[7] = W.w;
-    out[8] = W.prev.k;
-    for (int j = 0; j < 6; ++j) out[9 + j] = W.prev.a[j];
-    out[15] = 1.0;
+    for (int j = 0; j < 6; ++j) out[(1 + j) * kWinField] = W.form.a[j];
+    out[7 * kWinField] = W.w;
+    out[8 * kWinField] = W.prev.k;
+    for (int j = 0; j < 6; ++j) out[(9 + j) * kWinField] = W.prev.a[j];
+    int32_t nbj[4];
+    uint32_t stat, upper;
+    ray_neighbours(F, i, nbj, stat, upper);
+    out[15 * kWinField] = 1.0 + (double)stat;
+    const DPoint cur = dpoint(A, at(S.curr, F.N, i));
+    for (int n = 0; n < 4; ++n) {
+        double* o = out + (16 + 7 * n) * kWinField;
+        if ((stat >> n) & 1u) {
+            const LinForm L = lin_diff(dpoint(A, at(S.curr, F.N, nbj[n])), cur);
+            o[0] = L.k;
+            for (int j = 0; j < 6; ++j) o[(1 + j) * kWinField] = L.a[j];
+        } else {
+            for (int j = 0; j < 7; ++j) o[j * kWinField] = 0.0;
+        }
+    }
+}
+
+// use bits of a stored record: the AZ rule's bits, less the DE neighbours
+// on an edge of the curr slot
+WQ_HD_INLINE uint32_t cell_use(const FanConst& F, const Arrays& A, const Slots& S, int64_t i, uint32_t stat) {
+    uint32_t use = stat;
+    if ((use & 1u) && A.edge[at(S.curr, F.N, i - F.naz)] != 0) use &= ~1u;
+    if ((use & 2u) && A.edge[at(S.curr, F.N, i + F.naz)] != 0) use &= ~2u;
+    return use;
 }
 
 WQ_HD_INLINE bool load_window(const FanConst& F, const Arrays& A, int64_t i, DetectWindow& W) {
     if (i < 0 || i >= F.N) return false;
-    const double* in = A.win + i * 16;
-    if (in[15] == 0.0) return false;
+    const double* in = A.win + win_slot(F, i);
+    if (in[15 * kWinField] == 0.0) return false;
     W.form.k = in[0];
-    for (int j = 0; j < 6; ++j) W.form.a[j] = in[1 + j];
-    W.w = in[7];
-    W.prev.k = in[8];
-    for (int j = 0; j < 6; ++j) W.prev.a[j] = in[9 + j];
+    for (int j = 0; j < 6; ++j) W.form.a[j] = in[(1 + j) * kWinField];
+    W.w = in[7 * kWinField];
+    W.prev.k = in[8 * kWinField];
+    for (int j = 0; j < 6; ++j) W.prev.a[j] = in[(9 + j) * kWinField];
     W.m = F.lin_margin;
+    return true;
+}
+
+// the whole record of ray i as a DetectCell; false for rays not searched
+WQ_HD_INLINE bool load_cell(const FanConst& F, const Arrays& A, const Slots& S, int64_t i, DetectCell& C) {
+    if (!load_window(F, A, i, C.W)) return false;
+    const double* in = A.win + win_slot(F, i);
+    for (int n = 0; n < 4; ++n) {
+        C.nb[n].k = in[(16 + 7 * n) * kWinField];
+        for (int j = 0; j < 6; ++j) C.nb[n].a[j] = in[(17 + 7 * n + j) * kWinField];
+    }
+    C.use = cell_use(F, A, S, i, (uint32_t)in[15 * kWinField] - 1u);
     return true;
 }
 
@@ -909,32 +1052,14 @@ WQ_HD_INLINE bool detect_ray(const FanConst& F, const Arrays& A, const Slots& S,
                              DetectRay& r) {
     if (!detect_interior(F, A, S, i)) return false;
     const int64_t N = F.N;
-    const int de = (int)(i / F.naz), az = (int)(i % F.naz);
-    const int max_az = F.naz - 1;
     r.i = i;
     r.cur = dpoint(A, at(S.curr, N, i));
-    // neighbour indices with the AZ wrap of is_closest_ray (wave_queue.cc:423-431)
-    const int dd[4] = {de - 1, de + 1, de, de};
-    const int nz[4] = {1, 1, 0, 2};
-    const int nd[4] = {0, 2, 1, 1};
-    r.use = 0;
-    r.upper = 0;
-    for (int n = 0; n < 4; ++n) {
-        int a = az + nz[n] - 1;
-        if (F.az_boundary) {
-            if (az + nz[n] == 0) {
-                a = F.naz - 2;
-            } else if (a >= max_az) {
-                a = 0;
-            }
-        }
-        const int64_t j = (int64_t)dd[n] * F.naz + a;
-        bool use = !((a == 0 && !F.az_boundary) || a == max_az);
-        if (use && nd[n] != 1) use = A.edge[at(S.curr, N, j)] == 0;
-        r.nbj[n] = (int32_t)j;
-        r.use |= (use ? 1u : 0u) << n;
-        r.upper |= ((nd[n] == 2 || nz[n] == 2) ? 1u : 0u) << n;
-    }
+    uint32_t stat;
+    ray_neighbours(F, i, r.nbj, stat, r.upper);
+    // the DE neighbours (n = 0, 1) are skipped when on an edge
+    r.use = stat;
+    for (int n = 0; n < 2; ++n)
+        if (((stat >> n) & 1u) && A.edge[at(S.curr, N, r.nbj[n])] != 0) r.use &= ~(1u << n);
     return true;
 }
 
