@@ -57,13 +57,32 @@ constexpr int kXcds = 8;
 static_assert(kTargetBlock == 64, "k_detect tests one target per lane");
 
 #ifndef WQ3D_EMULATE
+// The per-ray kernels come in variants with the environment's kinds fixed
+// at compile time: PK the profile kind (-1: any), FLAT both boundaries
+// flat.  The kernel's own copy of the descriptor is given the constant
+// kinds, so the branches of the other kinds (the grid interpolation above
+// all) fold away, which shrinks the register footprint of the reflection
+// path.  launch_* picks the variant from the descriptor.
+template <int PK, bool FLAT>
+__device__ __forceinline__ void fix_kinds(DEnv& E) {
+    if (PK >= 0) E.profile.kind = PK;
+    if (FLAT) {
+        E.surface.kind = WQ3D_BOUNDARY_FLAT;
+        E.bottom.kind = WQ3D_BOUNDARY_FLAT;
+    }
+}
+
+template <int PK, bool FLAT>
 __global__ void __launch_bounds__(kBlock) k_init(DEnv E, FanConst F, Arrays A, Slots S) {
+    fix_kinds<PK, FLAT>(E);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < F.N) body_init(i, E, F, A, S);
 }
 
+template <int PK, bool FLAT>
 __global__ void __launch_bounds__(kBlock) k_reflect(DEnv E, FanConst F, Arrays A, Slots S,
                                                    Records Rc, double time, int64_t step_key) {
+    fix_kinds<PK, FLAT>(E);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // the eigenray search counters of this step (read by later launches)
     if (i == 0) {
@@ -73,7 +92,9 @@ __global__ void __launch_bounds__(kBlock) k_reflect(DEnv E, FanConst F, Arrays A
     if (i < F.N) body_reflect(i, E, F, A, S, Rc, time, step_key);
 }
 
+template <int PK, bool FLAT>
 __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A, Slots S) {
+    fix_kinds<PK, FLAT>(E);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < F.N) {
         body_advance(i, E, F, A, S);
@@ -362,10 +383,34 @@ inline hipError_t last_error() { return hipSuccess; }
 }  // namespace rt
 
 // kernel launches (serial loops over the same bodies in the emulation build)
+#ifndef WQ3D_EMULATE
+// the compile-time environment variant for E (see fix_kinds): 0 generic,
+// 1 Munk profile with flat boundaries, 2 linear profile with flat boundaries
+int env_variant(const DEnv& E) {
+    const bool flat = E.surface.kind == WQ3D_BOUNDARY_FLAT && E.bottom.kind == WQ3D_BOUNDARY_FLAT;
+    if (flat && E.profile.kind == WQ3D_PROFILE_MUNK) return 1;
+    if (flat && E.profile.kind == WQ3D_PROFILE_LINEAR) return 2;
+    return 0;
+}
+#define WQ3D_LAUNCH_ENV(kernel, E, grid, st, ...)                                              \
+    do {                                                                                       \
+        switch (env_variant(E)) {                                                              \
+        case 1:                                                                                \
+            hipLaunchKernelGGL((kernel<WQ3D_PROFILE_MUNK, true>), dim3(grid), dim3(kBlock), 0, st, E, __VA_ARGS__); \
+            break;                                                                             \
+        case 2:                                                                                \
+            hipLaunchKernelGGL((kernel<WQ3D_PROFILE_LINEAR, true>), dim3(grid), dim3(kBlock), 0, st, E, __VA_ARGS__); \
+            break;                                                                             \
+        default:                                                                               \
+            hipLaunchKernelGGL((kernel<-1, false>), dim3(grid), dim3(kBlock), 0, st, E, __VA_ARGS__); \
+        }                                                                                      \
+    } while (0)
+#endif
+
 void launch_init(hipStream_t st, unsigned grid, const DEnv& E, const FanConst& F, const Arrays& A,
                  const Slots& S) {
 #ifndef WQ3D_EMULATE
-    hipLaunchKernelGGL(k_init, dim3(grid), dim3(kBlock), 0, st, E, F, A, S);
+    WQ3D_LAUNCH_ENV(k_init, E, grid, st, F, A, S);
 #else
     for (int64_t i = 0; i < F.N; ++i) body_init(i, E, F, A, S);
 #endif
@@ -374,7 +419,7 @@ void launch_init(hipStream_t st, unsigned grid, const DEnv& E, const FanConst& F
 void launch_reflect(hipStream_t st, unsigned grid, const DEnv& E, const FanConst& F, const Arrays& A,
                     const Slots& S, const Records& R, double t, int64_t key) {
 #ifndef WQ3D_EMULATE
-    hipLaunchKernelGGL(k_reflect, dim3(grid), dim3(kBlock), 0, st, E, F, A, S, R, t, key);
+    WQ3D_LAUNCH_ENV(k_reflect, E, grid, st, F, A, S, R, t, key);
 #else
     *R.cand_count = 0ull;
     *R.hit_count = 0ull;
@@ -385,7 +430,7 @@ void launch_reflect(hipStream_t st, unsigned grid, const DEnv& E, const FanConst
 void launch_advance(hipStream_t st, unsigned grid, const DEnv& E, const FanConst& F, const Arrays& A,
                     const Slots& S) {
 #ifndef WQ3D_EMULATE
-    hipLaunchKernelGGL(k_advance, dim3(grid), dim3(kBlock), 0, st, E, F, A, S);
+    WQ3D_LAUNCH_ENV(k_advance, E, grid, st, F, A, S);
 #else
     for (int64_t i = 0; i < F.N; ++i) {
         body_advance(i, E, F, A, S);

@@ -318,7 +318,7 @@ __host__ __device__ double grid_interpolate(const DGrid& g, const double locatio
 // ------------------------------------------------------------- profiles
 // profile_munk.cc:19-32, profile_linear.cc:14-40, profile_grid.h:68-130,
 // profile_model::adjust_speed (profile_model.cc:14-26)
-__host__ __device__ inline void sound_speed(const DProfile& P, double rho, double theta, double phi,
+__host__ __device__ __forceinline__ void sound_speed(const DProfile& P, double rho, double theta, double phi,
                                    double R, double& c, double g[3], bool want_gradient) {
     double grho = 0.0, gtheta = 0.0, gphi = 0.0, speed;
     if (P.kind == 1) {
@@ -372,7 +372,7 @@ __host__ __device__ __forceinline__ double attenuation(const DProfile& P, double
 
 // ------------------------------------------------------------ boundaries
 // boundary_flat.cc:55-63, boundary_grid.h:145-196
-__host__ __device__ inline void boundary_height(const DBoundary& B, double theta, double phi, double& h,
+__host__ __device__ __forceinline__ void boundary_height(const DBoundary& B, double theta, double phi, double& h,
                                        double n[3], bool want_normal) {
     if (B.kind == 0) {
         h = B.rho;
@@ -466,7 +466,7 @@ struct Ray {
 };
 
 // wave_front::update minus attenuation (wave_front.cc:125-178,241-250)
-__host__ __device__ inline void update_ray(const DEnv& E, double R, Ray& w) {
+__host__ __device__ __forceinline__ void update_ray(const DEnv& E, double R, Ray& w) {
     double g[3];
     sound_speed(E.profile, w.pr, w.pt, w.pp, R, w.c, g, true);
     w.cr = g[0];
