@@ -2,12 +2,14 @@
 //
 // One wave_queue::step() (waveq3d/wave_queue.cc:155-192) is five launches on
 // the fan's stream:
-//   k_reflect  detect_reflections without find_edges (wave_queue.cc:197-292,
-//              reflection_model.cc:33-360): one thread per ray; the reference's
-//              de-major caustic dependency is a gather on the lower DE
-//              neighbour's pre-reflection state, which is read-only here
-//              because whether that neighbour reflects (flag[]) was decided
-//              by the previous k_advance.
+//   k_reflect  detect_reflections without find_edges (wave_queue.cc:197-292):
+//              one thread per ray for the caustic and vertex counters; the
+//              reference's de-major caustic dependency is a gather on the
+//              lower DE neighbour's pre-reflection state, which is read-only
+//              here because whether that neighbour reflects (flag[]) was
+//              decided by the previous k_advance.
+//   k_bounce   the reflections of the rays flag[] marks
+//              (reflection_model.cc:33-360).
 //   k_advance  find_edges on the old next slot (wave_front.cc:184-216, as a
 //              gather), queue rotation, AB3 + update() of the new next slot
 //              (ode_integ.cc:131-186, wave_front.cc:125-178), attenuation and
@@ -56,6 +58,16 @@ constexpr int kHitQueue = 128;
 constexpr int kXcds = 8;
 static_assert(kTargetBlock == 64, "k_detect tests one target per lane");
 
+// The parameters of a fan in device memory, for kernels whose per-ray body
+// is too large to inline: passing the kernel's by-value arguments on by
+// reference would copy them to scratch memory in every wavefront.
+struct KParams {
+    DEnv E;
+    FanConst F;
+    Arrays A;
+    Records R;
+};
+
 #ifndef WQ3D_EMULATE
 // The per-ray kernels come in variants with the environment's kinds fixed
 // at compile time: PK the profile kind (-1: any), FLAT both boundaries
@@ -79,17 +91,24 @@ __global__ void __launch_bounds__(kBlock) k_init(DEnv E, FanConst F, Arrays A, S
     if (i < F.N) body_init(i, E, F, A, S);
 }
 
-template <int PK, bool FLAT>
-__global__ void __launch_bounds__(kBlock) k_reflect(DEnv E, FanConst F, Arrays A, Slots S,
-                                                   Records Rc, double time, int64_t step_key) {
-    fix_kinds<PK, FLAT>(E);
+__global__ void __launch_bounds__(kBlock) k_reflect(FanConst F, Arrays A, Slots S, Records Rc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // the eigenray search counters of this step (read by later launches)
     if (i == 0) {
         *Rc.cand_count = 0ull;
         *Rc.hit_count = 0ull;
     }
-    if (i < F.N) body_reflect(i, E, F, A, S, Rc, time, step_key);
+    if (i < F.N) body_reflect_light(i, F, A, S);
+}
+
+// the reflecting rays (flag[] != 0; volume scattering needs every ray);
+// rows of the fan reflect together, so most wavefronts leave after one load
+__global__ void __launch_bounds__(kBlock) k_bounce(const KParams* __restrict__ P, Slots S, double time,
+                                                  int64_t step_key) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P->F.N) return;
+    if (P->A.flag[i] == 0 && !(P->F.want_verbs && P->E.nvol > 0)) return;
+    body_bounce(i, P->E, P->F, P->A, S, P->R, time, step_key);
 }
 
 template <int PK, bool FLAT>
@@ -416,14 +435,24 @@ void launch_init(hipStream_t st, unsigned grid, const DEnv& E, const FanConst& F
 #endif
 }
 
-void launch_reflect(hipStream_t st, unsigned grid, const DEnv& E, const FanConst& F, const Arrays& A,
-                    const Slots& S, const Records& R, double t, int64_t key) {
+void launch_reflect(hipStream_t st, unsigned grid, const FanConst& F, const Arrays& A, const Slots& S,
+                    const Records& R) {
 #ifndef WQ3D_EMULATE
-    WQ3D_LAUNCH_ENV(k_reflect, E, grid, st, F, A, S, R, t, key);
+    hipLaunchKernelGGL(k_reflect, dim3(grid), dim3(kBlock), 0, st, F, A, S, R);
 #else
     *R.cand_count = 0ull;
     *R.hit_count = 0ull;
-    for (int64_t i = 0; i < F.N; ++i) body_reflect(i, E, F, A, S, R, t, key);
+    for (int64_t i = 0; i < F.N; ++i) body_reflect_light(i, F, A, S);
+#endif
+}
+
+void launch_bounce(hipStream_t st, unsigned grid, const KParams* kp, const DEnv& E, const FanConst& F,
+                   const Arrays& A, const Slots& S, const Records& R, double t, int64_t key) {
+#ifndef WQ3D_EMULATE
+    hipLaunchKernelGGL(k_bounce, dim3(grid), dim3(kBlock), 0, st, kp, S, t, key);
+#else
+    (void)kp;
+    for (int64_t i = 0; i < F.N; ++i) body_bounce(i, E, F, A, S, R, t, key);
 #endif
 }
 
@@ -612,6 +641,8 @@ struct wq3d_env {
 
 struct wq3d_fan {
     wq3d_env* env = nullptr;
+    KParams* kp = nullptr;   // device copy of {env, F, A, R}
+    bool kp_dirty = true;    // F changed since the last upload
     int device = 0;
     hipStream_t stream = nullptr;
     DeviceBuffers mem;
@@ -773,7 +804,8 @@ hipEvent_t take_event(wq3d_fan* f) {
 }
 
 // kernel ids for timing
-enum { KID_INIT = 0, KID_REFLECT = 1, KID_ADVANCE = 2, KID_DETECT = 3, KID_BUILD = 4, KID_EXACT = 5, KID_COUNT = 6 };
+enum { KID_INIT = 0, KID_REFLECT = 1, KID_ADVANCE = 2, KID_DETECT = 3, KID_BUILD = 4, KID_EXACT = 5, KID_BOUNCE = 6,
+       KID_COUNT = 7 };
 
 template <class Launch>
 hipError_t timed(wq3d_fan* f, int kid, Launch&& launch) {
@@ -1181,7 +1213,8 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
             f->mem.alloc(&R.verb_power, R.verb_cap * F.nf) != hipSuccess ||
             f->mem.alloc(&R.refl, R.refl_cap) != hipSuccess ||
             f->mem.alloc(&R.cand, R.cand_cap) != hipSuccess || f->mem.alloc(&R.hit, R.hit_cap) != hipSuccess ||
-            f->mem.alloc(&f->counters, 5 + kScanSlots) != hipSuccess || f->mem.alloc(&f->error, 1) != hipSuccess)
+            f->mem.alloc(&f->counters, 5 + kScanSlots) != hipSuccess || f->mem.alloc(&f->error, 1) != hipSuccess ||
+            f->mem.alloc(&f->kp, 1) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "record allocation failed"));
         R.ray_count = f->counters + 0;
         R.verb_count = f->counters + 1;
@@ -1229,6 +1262,13 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
     const FanConst& F = f->F;
     const unsigned grid = (unsigned)((F.N + kBlock - 1) / kBlock);
     const DEnv& E = f->env->dev;
+    if (f->kp_dirty) {
+        // after the launches that read the previous copy
+        HIP_TRY(rt::sync(f->stream));
+        const KParams kp{E, F, f->A, f->R};
+        HIP_TRY(rt::memcpy(f->kp, &kp, sizeof(KParams), hipMemcpyHostToDevice));
+        f->kp_dirty = false;
+    }
     // the detect grid: 4 ray tiles (wavefronts) per workgroup times the
     // super-blocks of targets
     const int64_t tiles = ((F.nde + kTile - 1) / kTile) * ((F.naz + kTile - 1) / kTile);
@@ -1240,7 +1280,10 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
             const double t = f->time;
             const int64_t key = f->steps;
             HIP_TRY(timed(f, KID_REFLECT, [&] {
-                launch_reflect(f->stream, grid, E, F, f->A, S, f->R, t, key);
+                launch_reflect(f->stream, grid, F, f->A, S, f->R);
+            }));
+            HIP_TRY(timed(f, KID_BOUNCE, [&] {
+                launch_bounce(f->stream, grid, f->kp, E, F, f->A, S, f->R, t, key);
             }));
         }
         // rotate: past <- prev <- curr <- next <- past
@@ -1307,6 +1350,7 @@ wq3d_status wq3d_set_thresholds(wq3d_fan* f, const wq3d_thresholds* th) {
     f->F.max_caustic = th->max_caustic;
     f->F.max_upper = th->max_upper;
     f->F.max_lower = th->max_lower;
+    f->kp_dirty = true;
     return WQ3D_OK;
 }
 
@@ -1314,6 +1358,7 @@ wq3d_status wq3d_set_listeners(wq3d_fan* f, int32_t want_eigenverbs, int32_t wan
     if (!f) return fail(WQ3D_ERR_ARGUMENT, "null fan");
     f->F.want_verbs = want_eigenverbs != 0;
     f->F.want_refl = want_reflections != 0;
+    f->kp_dirty = true;
     return WQ3D_OK;
 }
 

@@ -571,10 +571,17 @@ WQ_HD bool try_bottom(RayCtx& x) {
     return true;
 }
 
-WQ_HD void body_reflect(int64_t i, const DEnv& E, const FanConst& F, const Arrays& A, const Slots& S,
-                       const Records& Rc, double time, int64_t step_key) {
+// detect_reflections (wave_queue.cc:197-292) in two parts.  The first runs
+// for every ray: the caustic test against the lower DE neighbour and, for
+// rays that do not reflect this step, the vertex counters.  The second
+// runs volume scattering and then, for the rays flag[] marks, the
+// reflections.  A ray's caustic test only reads a neighbour that does not
+// reflect, which the second part never writes; the caustic and vertex
+// updates touch the next slot's counters and phase, which volume
+// scattering does not read, so the split keeps the reference's results.
+WQ_HD void body_reflect_light(int64_t i, const FanConst& F, const Arrays& A, const Slots& S) {
     const int64_t N = F.N;
-    const int de = (int)(i / F.naz), az = (int)(i % F.naz);
+    const int de = (int)(i / F.naz);
     const int64_t kc = at(S.curr, N, i), kn = at(S.next, N, i);
     // caustic found while processing the lower DE neighbour, which precedes
     // this ray in the reference's de-major loop (wave_queue.cc:274-292)
@@ -590,20 +597,25 @@ WQ_HD void body_reflect(int64_t i, const DEnv& E, const FanConst& F, const Array
             for (int f = 0; f < F.nf; ++f) A.phase[((int64_t)S.next * F.nf + f) * N + i] -= M_PI_2;
         }
     }
+    if (A.flag[i] != 0) return;
+    // wave_queue::detect_vertices (wave_queue.cc:261-269)
+    const double L = A.nr[kc];
+    const double Rr = A.nr[kn];
+    if (L * Rr < 0.0 && Rr < 0.0) {
+        A.upr[kn]++;
+    } else if (L * Rr < 0.0 && 0.0 < Rr) {
+        A.lwr[kn]++;
+    }
+}
+
+WQ_HD void body_bounce(int64_t i, const DEnv& E, const FanConst& F, const Arrays& A, const Slots& S,
+                       const Records& Rc, double time, int64_t step_key) {
+    const bool reflects = A.flag[i] != 0;
+    if (!reflects && !(F.want_verbs && E.nvol > 0)) return;
+    const int de = (int)(i / F.naz), az = (int)(i % F.naz);
     RayCtx x{E, F, A, S, Rc, i, de, az, time, step_key, 0};
     volume_scattering(x);
-    const uint8_t kind = A.flag[i];
-    if (kind == 0) {
-        // wave_queue::detect_vertices (wave_queue.cc:261-269)
-        const double L = A.nr[kc];
-        const double Rr = A.nr[kn];
-        if (L * Rr < 0.0 && Rr < 0.0) {
-            A.upr[kn]++;
-        } else if (L * Rr < 0.0 && 0.0 < Rr) {
-            A.lwr[kn]++;
-        }
-        return;
-    }
+    if (!reflects) return;
     // surface <-> bottom recursion as a bounded loop
     bool bottom_next;
     if (try_surface(x)) {
