@@ -303,8 +303,9 @@ __global__ void __launch_bounds__(kBlock) k_exact(FanConst F, Arrays A, Slots S,
 // is_closest_ray + build_eigenray per candidate, grid-stride over the
 // device-side candidate count; each candidate is built by a group of G
 // consecutive lanes, one per frequency (body_build<true>)
-__global__ void __launch_bounds__(kBlock) k_build(DEnv E, FanConst F, Arrays A, Slots S, Records Rc,
-                                                 double time, int64_t step_key, int group) {
+__global__ void __launch_bounds__(kBlock) k_build(const KParams* __restrict__ P, Slots S, double time,
+                                                 int64_t step_key, int group) {
+    const Records& Rc = P->R;
     const unsigned long long raw = *Rc.cand_count;
     const int64_t ncand = (int64_t)(raw < (unsigned long long)Rc.cand_cap ? raw : (unsigned long long)Rc.cand_cap);
     // 27 neighbourhood values (or 2 x 27) per group of at least 4 lanes
@@ -313,7 +314,7 @@ __global__ void __launch_bounds__(kBlock) k_build(DEnv E, FanConst F, Arrays A, 
     const int lane_in_group = (int)(gid % group);
     double* lds = group >= 4 ? s_cube + (threadIdx.x / group) * 54 : nullptr;
     for (int64_t c = gid / group; c < ncand; c += (int64_t)gridDim.x * blockDim.x / group)
-        body_build<true>(c, lane_in_group, group, lds, E, F, A, S, Rc, time, step_key);
+        body_build<true>(c, lane_in_group, group, lds, P->E, P->F, P->A, S, Rc, time, step_key);
 }
 #endif  // WQ3D_EMULATE
 
@@ -538,13 +539,14 @@ void launch_exact(hipStream_t st, const FanConst& F, const Arrays& A, const Slot
 #endif
 }
 
-void launch_build(hipStream_t st, const DEnv& E, const FanConst& F, const Arrays& A, const Slots& S,
-                  const Records& R, double t, int64_t key) {
+void launch_build(hipStream_t st, const KParams* kp, const DEnv& E, const FanConst& F, const Arrays& A,
+                  const Slots& S, const Records& R, double t, int64_t key) {
 #ifndef WQ3D_EMULATE
     int group = kBuildLanes;  // a power of two, at least the frequency count
     while (group < F.nf) group <<= 1;
-    hipLaunchKernelGGL(k_build, dim3(1024), dim3(kBlock), 0, st, E, F, A, S, R, t, key, group);
+    hipLaunchKernelGGL(k_build, dim3(1024), dim3(kBlock), 0, st, kp, S, t, key, group);
 #else
+    (void)kp;
     const int64_t n = (int64_t)std::min<unsigned long long>(*R.cand_count, (unsigned long long)R.cand_cap);
     for (int64_t c = 0; c < n; ++c) body_build<false>(c, -1, 1, nullptr, E, F, A, S, R, t, key);
 #endif
@@ -1309,7 +1311,7 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
             const double t = f->time;
             const int64_t key = f->steps;
             HIP_TRY(timed(f, KID_BUILD, [&] {
-                launch_build(f->stream, E, F, f->A, S, f->R, t, key);
+                launch_build(f->stream, f->kp, E, F, f->A, S, f->R, t, key);
             }));
         }
         f->pending_steps += 1;
