@@ -278,7 +278,7 @@ wq3d_status wq3d_snapshot(wq3d_fan* fan, int32_t slot, wq3d_front_view* view);
 /* Average duration (ms) per launch of each kernel of the last step() calls,
  * measured with HIP events on the fan's stream when timing is enabled.
  * Kernel ids: 0 init, 1 reflect (caustics, vertices), 2 advance, 3 detect,
- * 4 build, 5 exact, 6 bounce (the reflections themselves).
+ * 4 build, 5 (not launched), 6 bounce (the reflections themselves).
  * enable: 0 off, 1 every kernel, 2 the eigenray search (detect) only, which
  * keeps the events' own cost out of the other launches. */
 wq3d_status wq3d_set_timing(wq3d_fan* fan, int32_t enable);
@@ -286,8 +286,10 @@ wq3d_status wq3d_kernel_times(wq3d_fan* fan, double* ms_per_launch, int64_t* lau
                               int32_t num_kernels);
 
 /* Diagnostics with no counterpart in the reference: the eigenray search's
- * pair counts of the last step (pairs inside the pre-test window, pairs that
- * passed the exact pre-test).  Synchronises the fan's stream. */
+ * pair counts of the last step (pairs the first-stage filter kept - the
+ * pre-test window and the neighbour forms - and pairs that passed the exact
+ * pre-test and neighbour comparison, the eigenray candidates).
+ * Synchronises the fan's stream. */
 wq3d_status wq3d_search_stats(wq3d_fan* fan, int64_t* window_hits, int64_t* candidates);
 
 /* Diagnostic with no counterpart in the reference: the (target, ray) pairs

@@ -14,13 +14,13 @@
 //              gather), queue rotation, AB3 + update() of the new next slot
 //              (ode_integ.cc:131-186, wave_front.cc:125-178), attenuation and
 //              counter carry (wave_queue.cc:174-183), next step's flag[].
-//   k_detect   eigenray time-CPA pre-test (wave_queue.cc:372-395) for every
-//              (target, interior ray) pair, first as a window on one linear
-//              form of the target distances (wave_front.cc:222-236), which
-//              rejects almost every pair; the rest go to a hit list.
-//   k_exact    the exact pre-test of every hit, plus the current-time part
-//              of is_closest_ray for the in-plane neighbours; survivors go
-//              to the candidate list.
+//   k_detect   the eigenray search (wave_queue.cc:372-398, 411-478): for
+//              every (target, interior ray) pair a filter of linear forms in
+//              the target's features - the time-CPA pre-test as a window and
+//              the current-time comparison with the four in-plane
+//              neighbours - first per block of targets, then per pair; the
+//              few pairs it keeps get the exact pre-test and neighbour
+//              comparison, and the survivors go to the candidate list.
 //   k_build    is_closest_ray + build_eigenray (wave_queue.cc:411-702) with
 //              the hybrid Gaussian spreading walk per candidate.
 // Data layout: structure of arrays, slot-major ([slot][ray], ray = de*naz+az
@@ -97,6 +97,7 @@ __global__ void __launch_bounds__(kBlock) k_reflect(FanConst F, Arrays A, Slots 
     if (i == 0) {
         *Rc.cand_count = 0ull;
         *Rc.hit_count = 0ull;
+        *Rc.pass_count = 0ull;
     }
     if (i < F.N) body_reflect_light(i, F, A, S);
 }
@@ -201,25 +202,27 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
     unsigned long long keep[kSuperBlock];
     for (int q = 0; q < kSuperBlock; ++q)
         keep[q] = __ballot(active && q < nbk && block_cell_ok(C, box_at(q), s_br[q] != 0));
-    unsigned qn = 0;                // keys in this wavefront's buffer
+    unsigned qn = 0;                // candidate keys in this wavefront's buffer
     unsigned long long tested = 0;  // (target, ray) pairs put to the pair test
+    unsigned long long hits = 0;    // pairs the filter kept
     auto flush = [&]() {
         wave_sync();
         unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(Rc.hit_count, (unsigned long long)qn);
+        if (lane == 0) base = atomicAdd(Rc.cand_count, (unsigned long long)qn);
         base = __shfl(base, 0);
         for (unsigned e = lane; e < qn; e += 64) {
             const unsigned long long at_ = base + e;
-            if ((int64_t)at_ < Rc.hit_cap)
-                Rc.hit[at_] = s_hit[wv][e];
+            if ((int64_t)at_ < Rc.cand_cap)
+                Rc.cand[at_] = s_hit[wv][e];
             else
                 atomicOr(Rc.error, 2);
         }
         qn = 0;
         wave_sync();
     };
-    // one target's pair test for every lane's ray; the later forms only
-    // while some lane is still in
+    // one target's pair test for every lane's ray, the later forms only
+    // while some lane is still in; the pairs the filter keeps go to the
+    // candidate list, whose exact tests k_build makes
     auto visit = [&](bool in, const double* f, int64_t t, bool bbr) {
         if (!__any(in)) return;
         in = in && detect_window_prev_ok(C.W, f);
@@ -229,10 +232,11 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
         const unsigned long long hm = __ballot(in);
         if (!hm) return;
         const unsigned cnt = (unsigned)__popcll(hm);
+        hits += cnt;
         if (qn + cnt > (unsigned)kHitQueue) flush();
         if (in)
             s_hit[wv][qn + (unsigned)__popcll(hm & ((1ull << lane) - 1ull))] =
-                ((unsigned long long)t << 32) | (unsigned long long)ray;
+                ((unsigned long long)F.tgt_perm[t] << 32) | (unsigned long long)ray;
         qn += cnt;
     };
     const ConstLinFeatures* cf = (const ConstLinFeatures*)(F.tgt_lin);
@@ -265,38 +269,9 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
         }
     }
     if (qn) flush();
-    if (lane == 0 && tested) atomicAdd(&Rc.scan[blockIdx.x % kScanSlots], tested);
-}
-
-// Eigenray search, second stage: the exact pre-test (detect_pair) of every
-// hit, one thread per hit (grid-stride over the device-side count);
-// survivors are appended to the candidate list one atomic per wavefront.
-__global__ void __launch_bounds__(kBlock) k_exact(FanConst F, Arrays A, Slots S, Records Rc) {
-    const unsigned long long raw = *Rc.hit_count;
-    const int64_t nhit = (int64_t)(raw < (unsigned long long)Rc.hit_cap ? raw : (unsigned long long)Rc.hit_cap);
-    const ArrayPoints pts{A, S, F.N};
-    const int lane = threadIdx.x & 63;
-    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < nhit;
-         h += (int64_t)gridDim.x * blockDim.x) {
-        const unsigned long long hk = Rc.hit[h];
-        const int64_t t = (int64_t)F.tgt_perm[hk >> 32];
-        const int64_t i = (int64_t)(hk & 0xffffffffull);
-        const unsigned long long key = ((unsigned long long)t << 32) | (unsigned long long)i;
-        DetectRay r;
-        const bool pass = detect_ray(F, A, S, i, r) && detect_pair(r, pts, dtarget(F, t), F.tgt_branch[t] != 0);
-        const unsigned long long mask = __ballot(pass);
-        if (!mask) continue;
-        const int leader = __ffsll((long long)mask) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(Rc.cand_count, (unsigned long long)__popcll(mask));
-        base = __shfl(base, leader);
-        if (pass) {
-            const unsigned long long at_ = base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
-            if ((int64_t)at_ < Rc.cand_cap)
-                Rc.cand[at_] = key;
-            else
-                atomicOr(Rc.error, 2);
-        }
+    if (lane == 0) {
+        if (tested) atomicAdd(&Rc.scan[blockIdx.x % kScanSlots], tested);
+        if (hits) atomicAdd(Rc.hit_count, hits);
     }
 }
 
@@ -443,6 +418,7 @@ void launch_reflect(hipStream_t st, unsigned grid, const FanConst& F, const Arra
 #else
     *R.cand_count = 0ull;
     *R.hit_count = 0ull;
+    *R.pass_count = 0ull;
     for (int64_t i = 0; i < F.N; ++i) body_reflect_light(i, F, A, S);
 #endif
 }
@@ -497,9 +473,11 @@ void launch_detect(hipStream_t st, int64_t quads, const FanConst& F, const Array
             R.scan[0] += (unsigned long long)(t_end - b * kTargetBlock);
             for (int64_t t = b * kTargetBlock; t < t_end; ++t) {
                 if (!detect_cell_ok(C, F.tgt_lin + t * 6, F.scan_branch[t] != 0)) continue;
-                const unsigned long long off = (*R.hit_count)++;
-                if ((int64_t)off < R.hit_cap)
-                    R.hit[off] = ((unsigned long long)t << 32) | (unsigned long long)i;
+                *R.hit_count += 1;
+                const int64_t tp = (int64_t)F.tgt_perm[t];
+                const unsigned long long off = (*R.cand_count)++;
+                if ((int64_t)off < R.cand_cap)
+                    R.cand[off] = ((unsigned long long)tp << 32) | (unsigned long long)i;
                 else
                     *R.error |= 2;
             }
@@ -508,34 +486,9 @@ void launch_detect(hipStream_t st, int64_t quads, const FanConst& F, const Array
     }
     if (trace)
         std::fprintf(stderr, "[emu] rays searched %lld, kept by a super-block box %lld; ray-block pairs kept: "
-                     "window %.4f, cell %.5f; hits %llu\n", (long long)rays, (long long)rays_kept,
+                     "window %.4f, cell %.5f; hits %llu, candidates %llu\n", (long long)rays, (long long)rays_kept,
                      (double)window_blocks / (double)std::max<int64_t>(1, ray_blocks),
-                     (double)cell_blocks / (double)std::max<int64_t>(1, ray_blocks), *R.hit_count);
-#endif
-}
-
-void launch_exact(hipStream_t st, const FanConst& F, const Arrays& A, const Slots& S, const Records& R) {
-#ifndef WQ3D_EMULATE
-    hipLaunchKernelGGL(k_exact, dim3(2048), dim3(kBlock), 0, st, F, A, S, R);
-#else
-    const ArrayPoints pts{A, S, F.N};
-    const int64_t n = (int64_t)std::min<unsigned long long>(*R.hit_count, (unsigned long long)R.hit_cap);
-    if (std::getenv("WQ3D_EMU_TRACE"))
-        std::fprintf(stderr, "[emu] hits %lld (cap %lld), candidates before %llu\n", (long long)*R.hit_count,
-                     (long long)R.hit_cap, *R.cand_count);
-    for (int64_t h = 0; h < n; ++h) {
-        const int64_t t = (int64_t)F.tgt_perm[R.hit[h] >> 32];
-        const int64_t i = (int64_t)(R.hit[h] & 0xffffffffull);
-        const unsigned long long key = ((unsigned long long)t << 32) | (unsigned long long)i;
-        DetectRay r;
-        if (!detect_ray(F, A, S, i, r) || !detect_pair(r, pts, dtarget(F, t), F.tgt_branch[t] != 0)) continue;
-        const unsigned long long off = (*R.cand_count)++;
-        if ((int64_t)off < R.cand_cap)
-            R.cand[off] = key;
-        else
-            *R.error |= 2;
-    }
-    if (std::getenv("WQ3D_EMU_TRACE")) std::fprintf(stderr, "[emu] candidates %llu\n", *R.cand_count);
+                     (double)cell_blocks / (double)std::max<int64_t>(1, ray_blocks), *R.hit_count, *R.cand_count);
 #endif
 }
 
@@ -1206,7 +1159,6 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         R.ray_cap = std::max<int64_t>(1 << 16, std::min<int64_t>(nt * 64, 1 << 22));
         R.verb_cap = std::max<int64_t>(1 << 16, std::min<int64_t>(N * 4, 1 << 22));
         R.refl_cap = std::max<int64_t>(1 << 16, std::min<int64_t>(N * 4, 1 << 22));
-        R.hit_cap = nt > 0 ? std::max<int64_t>(1 << 20, std::min<int64_t>(nt * N / 8, 1 << 27)) : 1;
         R.cand_cap = nt > 0 ? std::max<int64_t>(1 << 20, std::min<int64_t>(nt * N / 64, 1 << 24)) : 1;
         if (f->mem.alloc(&R.rays, R.ray_cap) != hipSuccess ||
             f->mem.alloc(&R.ray_int, R.ray_cap * F.nf) != hipSuccess ||
@@ -1214,8 +1166,8 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
             f->mem.alloc(&R.verbs, R.verb_cap) != hipSuccess ||
             f->mem.alloc(&R.verb_power, R.verb_cap * F.nf) != hipSuccess ||
             f->mem.alloc(&R.refl, R.refl_cap) != hipSuccess ||
-            f->mem.alloc(&R.cand, R.cand_cap) != hipSuccess || f->mem.alloc(&R.hit, R.hit_cap) != hipSuccess ||
-            f->mem.alloc(&f->counters, 5 + kScanSlots) != hipSuccess || f->mem.alloc(&f->error, 1) != hipSuccess ||
+            f->mem.alloc(&R.cand, R.cand_cap) != hipSuccess ||
+            f->mem.alloc(&f->counters, 6 + kScanSlots) != hipSuccess || f->mem.alloc(&f->error, 1) != hipSuccess ||
             f->mem.alloc(&f->kp, 1) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "record allocation failed"));
         R.ray_count = f->counters + 0;
@@ -1223,9 +1175,10 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         R.refl_count = f->counters + 2;
         R.cand_count = f->counters + 3;
         R.hit_count = f->counters + 4;
-        R.scan = f->counters + 5;
+        R.pass_count = f->counters + 5;
+        R.scan = f->counters + 6;
         R.error = f->error;
-        if (rt::memset(f->counters, 0, (5 + kScanSlots) * sizeof(unsigned long long)) != hipSuccess ||
+        if (rt::memset(f->counters, 0, (6 + kScanSlots) * sizeof(unsigned long long)) != hipSuccess ||
             rt::memset(f->error, 0, sizeof(int32_t)) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "memset failed"));
     }
@@ -1307,7 +1260,6 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
             HIP_TRY(timed(f, KID_DETECT, [&] {
                 launch_detect(f->stream, quads, F, f->A, S, f->R);
             }));
-            HIP_TRY(timed(f, KID_EXACT, [&] { launch_exact(f->stream, F, f->A, S, f->R); }));
             const double t = f->time;
             const int64_t key = f->steps;
             HIP_TRY(timed(f, KID_BUILD, [&] {
@@ -1495,9 +1447,9 @@ wq3d_status wq3d_search_stats(wq3d_fan* f, int64_t* window_hits, int64_t* candid
     if (!f || !window_hits || !candidates) return fail(WQ3D_ERR_ARGUMENT, "null argument");
     HIP_TRY(rt::sync(f->stream));
     unsigned long long c[2] = {0, 0};
-    if (f->R.cand_count) HIP_TRY(rt::memcpy(c, f->R.cand_count, sizeof(c), hipMemcpyDeviceToHost));
-    *candidates = (int64_t)c[0];
-    *window_hits = (int64_t)c[1];
+    if (f->R.hit_count) HIP_TRY(rt::memcpy(c, f->R.hit_count, sizeof(c), hipMemcpyDeviceToHost));
+    *window_hits = (int64_t)c[0];
+    *candidates = (int64_t)c[1];
     return WQ3D_OK;
 }
 

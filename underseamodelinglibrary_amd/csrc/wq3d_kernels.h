@@ -115,10 +115,9 @@ struct Records {
     wq3d_reflection* refl;
     unsigned long long* refl_count;
     int64_t refl_cap;
-    unsigned long long* hit;    // (target, ray) pairs inside the pre-test window
-    unsigned long long* hit_count;
-    int64_t hit_cap;
-    unsigned long long* cand;   // pairs that passed the exact pre-test
+    unsigned long long* hit_count;  // pairs the first-stage filter kept, this step
+    unsigned long long* pass_count; // pairs that passed is_closest_ray, this step
+    unsigned long long* cand;   // pairs the first-stage filter kept: (target << 32) | ray
     unsigned long long* cand_count;
     int64_t cand_cap;
     unsigned long long* scan;   // [kScanSlots] (target, ray) pairs put to the
@@ -1784,7 +1783,10 @@ WQ_HD_INLINE void body_build(int64_t c, int fsel, int G, double* lds, const DEnv
         });
         if (Split) differ = group_any(differ, G);
         const double(&D)[3][3][3] = DE[0];
+        // the time-CPA pre-test (wave_queue.cc:384-394), then is_closest_ray
+        if (!(D[2][1][1] > D[1][1][1]) || D[0][1][1] < D[1][1][1]) return;
         if (!is_closest_ray(b, t, de, az, D, DE[1][1])) return;
+        if (!Split || fsel == 0) atom_add(Rc.pass_count, 1ull);
         // build_eigenray (wave_queue.cc:495-702)
         const int64_t ic = at(S.curr, F.N, i);
         if (above_bounce(F, A, ic)) return;
