@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 W_TGT = 13.0              # FP64 ops per (target, ray) pair of the window test: 6 FMA + 1 compare
+W_BOX = 59.0              # FP64 ops per (ray, box) bound: the window form bounded over the box, 2 compares
 # HBM bytes per k_detect launch at this workload: rocprofv3 FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0 (factors from a 1 GiB device copy), means of 310 launches;
 # profiles/k_detect_single_fan_115k_rays_10k_targets.md holds the commands
@@ -83,8 +84,8 @@ def main():
     q.drain_eigenrays()
     lib.check(lib.wq3d_set_timing(q._fan, 2))  # HIP events around k_detect only
     lib.check(lib.wq3d_kernel_times(q._fan, (C.c_double * 8)(), (C.c_int64 * 8)(), 8))
-    tested = C.c_int64()
-    lib.check(lib.wq3d_search_totals(q._fan, C.byref(tested)))  # restart the pair count
+    tested, boxes = C.c_int64(), C.c_int64()  # pairs tested, (ray, box) bounds tested
+    lib.check(lib.wq3d_search_totals(q._fan, C.byref(tested), C.byref(boxes)))  # restart the counts
     q.sync()
     t0 = time.perf_counter()
     done = 0
@@ -99,18 +100,20 @@ def main():
     ms = (C.c_double * 8)()
     launches = (C.c_int64 * 8)()
     lib.check(lib.wq3d_kernel_times(q._fan, ms, launches, 8))
-    lib.check(lib.wq3d_search_totals(q._fan, C.byref(tested)))
+    lib.check(lib.wq3d_search_totals(q._fan, C.byref(tested), C.byref(boxes)))
     rays, _, _ = q.drain_eigenrays()
     n_rays = sc.num_rays()
     value = n_rays * args.steps / elapsed
     kernel_ms = {KERNELS[k]: ms[k] for k in range(len(KERNELS)) if launches[k]}
-    # dominant kernel: the target search.  Its work is the window test of
-    # every (target, ray) pair in a target block that the per-block bound
-    # kept, counted on the device over the timed launches.
+    # dominant kernel: the target search.  Its work is the (ray, box) bounds
+    # it tests and the window test of every (target, ray) pair in a block
+    # the bounds kept, both counted on the device over the timed launches
+    # (the neighbour-form bounds a box test may add are not counted).
     det_ms = ms[3]
     det_launches = launches[3]
     pairs_per_launch = tested.value / det_launches if det_launches else 0.0
-    flops = W_TGT * pairs_per_launch
+    boxes_per_launch = boxes.value / det_launches if det_launches else 0.0
+    flops = W_TGT * pairs_per_launch + W_BOX * boxes_per_launch
     achieved = flops / (det_ms * 1e-3) / 1e12 if det_ms > 0 else 0.0
     out = {
         "metric": "ray-steps/sec + eigenrays/sec on 1 MI355X, plain vs batched+adjoint at fixed tol",
@@ -137,11 +140,12 @@ def main():
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": K_DETECT_TRAFFIC_BYTES,
                      "pairs_tested_per_launch": pairs_per_launch,
+                     "box_tests_per_launch": boxes_per_launch,
                      "pairs_all_per_launch": float(sc.num_targets() * n_rays),
-                     "note": "FP64 vector (no MFMA-shaped work); achieved = 13 FP64 ops (6 FMA + "
-                             "1 compare) per (target, ray) pair put to the window test, pairs "
-                             "counted on the device, / mean launch time (HIP events); traffic in "
-                             "bytes per launch"},
+                     "note": "FP64 vector (no MFMA-shaped work); achieved = (13 FP64 ops per "
+                             "(target, ray) pair put to the window test + 59 per (ray, box) bound "
+                             "tested), both counted on the device, / mean launch time (HIP "
+                             "events); traffic in bytes per launch"},
     }
     q.close()
     if not args.no_cpu_baseline:

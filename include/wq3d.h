@@ -292,11 +292,13 @@ wq3d_status wq3d_kernel_times(wq3d_fan* fan, double* ms_per_launch, int64_t* lau
  * Synchronises the fan's stream. */
 wq3d_status wq3d_search_stats(wq3d_fan* fan, int64_t* window_hits, int64_t* candidates);
 
-/* Diagnostic with no counterpart in the reference: the (target, ray) pairs
- * put to the eigenray search's window test (pairs in target blocks the
- * whole-block test did not rule out) since the fan was created or since the
- * last call, which resets the count.  Synchronises the fan's stream. */
-wq3d_status wq3d_search_totals(wq3d_fan* fan, int64_t* pairs_tested);
+/* Diagnostic with no counterpart in the reference: the work of the eigenray
+ * search's filter since the fan was created or since the last call, which
+ * resets the counts: the (target, ray) pairs put to the window test (pairs
+ * in target blocks the whole-block bounds did not rule out) and the
+ * (ray, box of targets) whole-block bounds tested.  Synchronises the fan's
+ * stream. */
+wq3d_status wq3d_search_totals(wq3d_fan* fan, int64_t* pairs_tested, int64_t* box_tests);
 
 #ifdef __cplusplus
 }

@@ -134,19 +134,26 @@ def test_detect_prefilter_matches_exact_comparisons():
 
 def test_search_totals_count_and_reset():
     """wq3d_search_totals: the pairs put to the window test over a run lie
-    between the last step's window hits and rays x targets x steps, and a
-    second call reads zero."""
+    between the last step's filter hits and rays x targets x steps, the
+    block bounds tested within one per (ray, step, box), and a second call
+    reads zero."""
     out = _run("""
         import ctypes as C
         sc = scenarios.config1(n_targets=(4, 4), time_max=3.0)
         g = wave_queue(*sc.args(), lib=lib)
         g._lib.check(g._lib.wq3d_step(g._fan, sc.steps))
-        h, c, t, t2 = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        h, c, t, b, t2, b2 = (C.c_int64() for _ in range(6))
         lib.check(lib.wq3d_search_stats(g._fan, C.byref(h), C.byref(c)))
-        lib.check(lib.wq3d_search_totals(g._fan, C.byref(t)))
-        lib.check(lib.wq3d_search_totals(g._fan, C.byref(t2)))
-        print("counts", h.value, c.value, t.value, t2.value, sc.num_rays() * sc.num_targets() * sc.steps)
+        lib.check(lib.wq3d_search_totals(g._fan, C.byref(t), C.byref(b)))
+        lib.check(lib.wq3d_search_totals(g._fan, C.byref(t2), C.byref(b2)))
+        nblk = (sc.num_targets() + 63) // 64
+        print("counts", h.value, c.value, t.value, b.value, t2.value, b2.value,
+              sc.num_rays() * sc.num_targets() * sc.steps,
+              sc.num_rays() * sc.steps * (nblk + (nblk + 7) // 8))
     """)
-    hits, cands, tested, again, bound = (int(x) for x in out.split("counts")[-1].split())
+    hits, cands, tested, forms, again, again_forms, bound, forms_bound = (
+        int(x) for x in out.split("counts")[-1].split())
     assert cands <= hits <= tested and 0 < tested < bound
-    assert again == 0
+    # at most one bound per (ray, step, box)
+    assert 0 < forms <= forms_bound
+    assert again == 0 and again_forms == 0

@@ -55,7 +55,7 @@ def bind(lib):
         "wq3d_set_timing": ([vp, i32], i32),
         "wq3d_kernel_times": ([vp, dp, C.POINTER(i64), i32], i32),
         "wq3d_search_stats": ([vp, C.POINTER(i64), C.POINTER(i64)], i32),
-        "wq3d_search_totals": ([vp, C.POINTER(i64)], i32),
+        "wq3d_search_totals": ([vp, C.POINTER(i64), C.POINTER(i64)], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)

@@ -196,7 +196,12 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
     };
     DetectCell C;
     bool active = load_cell(F, A, S, ray, C);
+    // (ray, box) bounds tested: the super-block's for every searched ray,
+    // each block's for the rays the super-block's bound kept
+    unsigned long long boxes_tested = (unsigned long long)__popcll(__ballot(active));
     active = active && block_cell_ok(C, box_at(kSuperBlock), s_br[kSuperBlock] != 0);
+    boxes_tested += (unsigned long long)__popcll(__ballot(active)) * (unsigned long long)nbk;
+    if (lane == 0 && boxes_tested) atomicAdd(&Rc.scan[(blockIdx.x % kScanSlots) * 2 + 1], boxes_tested);
     if (!__any(active)) return;  // wave-uniform; no barriers below
     // the blocks each ray keeps, decided for all blocks before any pair test
     unsigned long long keep[kSuperBlock];
@@ -270,7 +275,7 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
     }
     if (qn) flush();
     if (lane == 0) {
-        if (tested) atomicAdd(&Rc.scan[blockIdx.x % kScanSlots], tested);
+        if (tested) atomicAdd(&Rc.scan[(blockIdx.x % kScanSlots) * 2], tested);
         if (hits) atomicAdd(Rc.hit_count, hits);
     }
 }
@@ -465,8 +470,16 @@ void launch_detect(hipStream_t st, int64_t quads, const FanConst& F, const Array
         bool kept = false;
         for (int64_t b = 0; b < F.nblk; ++b) {
             const int64_t u = b / kSuperBlock;
-            if (!block_cell_ok(C, boxes[F.nblk + u], F.blk_branch[F.nblk + u] != 0)) continue;
+            // the super-block's bound, once per super-block
+            if (b % kSuperBlock == 0) {
+                R.scan[1] += 1;
+                if (!block_cell_ok(C, boxes[F.nblk + u], F.blk_branch[F.nblk + u] != 0)) {
+                    b += kSuperBlock - 1;
+                    continue;
+                }
+            }
             kept = true;
+            R.scan[1] += 1;
             if (!block_cell_ok(C, boxes[b], F.blk_branch[b] != 0)) continue;
             cell_blocks += 1;
             const int64_t t_end = std::min<int64_t>(F.nt, (b + 1) * kTargetBlock);
@@ -1167,7 +1180,7 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
             f->mem.alloc(&R.verb_power, R.verb_cap * F.nf) != hipSuccess ||
             f->mem.alloc(&R.refl, R.refl_cap) != hipSuccess ||
             f->mem.alloc(&R.cand, R.cand_cap) != hipSuccess ||
-            f->mem.alloc(&f->counters, 6 + kScanSlots) != hipSuccess || f->mem.alloc(&f->error, 1) != hipSuccess ||
+            f->mem.alloc(&f->counters, 6 + 2 * kScanSlots) != hipSuccess || f->mem.alloc(&f->error, 1) != hipSuccess ||
             f->mem.alloc(&f->kp, 1) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "record allocation failed"));
         R.ray_count = f->counters + 0;
@@ -1178,7 +1191,7 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         R.pass_count = f->counters + 5;
         R.scan = f->counters + 6;
         R.error = f->error;
-        if (rt::memset(f->counters, 0, (6 + kScanSlots) * sizeof(unsigned long long)) != hipSuccess ||
+        if (rt::memset(f->counters, 0, (6 + 2 * kScanSlots) * sizeof(unsigned long long)) != hipSuccess ||
             rt::memset(f->error, 0, sizeof(int32_t)) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "memset failed"));
     }
@@ -1453,14 +1466,18 @@ wq3d_status wq3d_search_stats(wq3d_fan* f, int64_t* window_hits, int64_t* candid
     return WQ3D_OK;
 }
 
-wq3d_status wq3d_search_totals(wq3d_fan* f, int64_t* pairs_tested) {
-    if (!f || !pairs_tested) return fail(WQ3D_ERR_ARGUMENT, "null argument");
+wq3d_status wq3d_search_totals(wq3d_fan* f, int64_t* pairs_tested, int64_t* box_tests) {
+    if (!f || !pairs_tested || !box_tests) return fail(WQ3D_ERR_ARGUMENT, "null argument");
     HIP_TRY(rt::sync(f->stream));
     *pairs_tested = 0;
+    *box_tests = 0;
     if (!f->R.scan) return WQ3D_OK;
-    unsigned long long c[kScanSlots];
+    unsigned long long c[2 * kScanSlots];
     HIP_TRY(rt::memcpy(c, f->R.scan, sizeof(c), hipMemcpyDeviceToHost));
-    for (int k = 0; k < kScanSlots; ++k) *pairs_tested += (int64_t)c[k];
+    for (int k = 0; k < kScanSlots; ++k) {
+        *pairs_tested += (int64_t)c[2 * k];
+        *box_tests += (int64_t)c[2 * k + 1];
+    }
     HIP_TRY(rt::memset_async(f->R.scan, 0, sizeof(c), f->stream));
     return WQ3D_OK;
 }

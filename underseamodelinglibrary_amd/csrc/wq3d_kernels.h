@@ -120,8 +120,9 @@ struct Records {
     unsigned long long* cand;   // pairs the first-stage filter kept: (target << 32) | ray
     unsigned long long* cand_count;
     int64_t cand_cap;
-    unsigned long long* scan;   // [kScanSlots] (target, ray) pairs put to the
-                                // window test, summed over steps
+    unsigned long long* scan;   // [kScanSlots][2] (target, ray) pairs put to the
+                                // window test and (ray, box) bounds tested,
+                                // summed over steps
     int32_t* error;             // bit 0: record overflow, bit 1: candidate overflow,
                                 // bit 2: reflection chain exceeded its bound
 };
