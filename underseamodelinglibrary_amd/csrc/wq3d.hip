@@ -318,6 +318,7 @@ inline hipError_t event_create(hipEvent_t* e) { return hipEventCreate(e); }
 inline hipError_t event_record(hipEvent_t e, hipStream_t s) { return hipEventRecord(e, s); }
 inline hipError_t event_elapsed(float* ms, hipEvent_t a, hipEvent_t b) { return hipEventElapsedTime(ms, a, b); }
 inline hipError_t event_destroy(hipEvent_t e) { return hipEventDestroy(e); }
+inline hipError_t event_sync(hipEvent_t e) { return hipEventSynchronize(e); }
 inline hipError_t last_error() { return hipGetLastError(); }
 #else
 // device allocations are not zeroed on the GPU: fill with 0xff (NaN doubles,
@@ -378,6 +379,7 @@ inline hipError_t event_elapsed(float* ms, hipEvent_t, hipEvent_t) {
     return hipSuccess;
 }
 inline hipError_t event_destroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t event_sync(hipEvent_t) { return hipSuccess; }
 inline hipError_t last_error() { return hipSuccess; }
 #endif
 }  // namespace rt
@@ -609,7 +611,7 @@ struct wq3d_env {
 
 struct wq3d_fan {
     wq3d_env* env = nullptr;
-    KParams* kp = nullptr;   // device copy of {env, F, A, R}
+    KParams* kp = nullptr;   // device copies of {env, F, A, R}, one per record set
     bool kp_dirty = true;    // F changed since the last upload
     int device = 0;
     hipStream_t stream = nullptr;
@@ -628,9 +630,23 @@ struct wq3d_fan {
     std::vector<wq3d_eigenverb> h_verbs;
     std::vector<double> h_verb_power;
     std::vector<wq3d_reflection> h_refl;
-    unsigned long long* counters = nullptr;  // device: ray, verb, refl, cand, hit
+    unsigned long long* counters = nullptr;  // device: ray, verb, refl, cand, hit, pass, scan[]
     int32_t* error = nullptr;
-    int64_t pending_steps = 0;  // steps since the last device->host record move
+    // Eigenray / eigenverb / reflection records go to two device buffer sets
+    // in turn, 64 steps each: while the kernels fill one, the host copies
+    // the other out, so the copy overlaps the next steps' kernels.
+    struct RecSet {
+        wq3d_eigenray* rays = nullptr;
+        double *ray_int = nullptr, *ray_phase = nullptr;
+        wq3d_eigenverb* verbs = nullptr;
+        double* verb_power = nullptr;
+        wq3d_reflection* refl = nullptr;
+        unsigned long long* count = nullptr;  // ray, verb, refl
+        hipEvent_t done = nullptr;            // after the last step that wrote the set
+        bool pending = false;                 // holds records not yet copied out
+    } rs[2];
+    int rset = 0;               // the set the kernels write
+    int64_t pending_steps = 0;  // steps written to the active set
     // timing
     int timing = 0;  // 0 off, 1 every kernel, 2 k_detect only
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
@@ -796,11 +812,31 @@ hipError_t timed(wq3d_fan* f, int kid, Launch&& launch) {
 Slots slots_of(const wq3d_fan* f) { return Slots{f->phys[0], f->phys[1], f->phys[2], f->phys[3]}; }
 
 // move device records to host vectors; device counters restart at zero
-wq3d_status collect_records(wq3d_fan* f) {
-    HIP_TRY(rt::sync(f->stream));
-    unsigned long long cnt[4];
+// point the kernels' record pointers (f->R and the KParams of the set) at
+// record set `s`
+void use_record_set(wq3d_fan* f, int s) {
+    const wq3d_fan::RecSet& r = f->rs[s];
+    Records& R = f->R;
+    R.rays = r.rays;
+    R.ray_int = r.ray_int;
+    R.ray_phase = r.ray_phase;
+    R.ray_count = r.count + 0;
+    R.verbs = r.verbs;
+    R.verb_power = r.verb_power;
+    R.verb_count = r.count + 1;
+    R.refl = r.refl;
+    R.refl_count = r.count + 2;
+    f->rset = s;
+}
+
+// copy record set `s` to the host vectors (its last writer has finished,
+// see `done`) and clear its counters on the stream, before any later launch
+wq3d_status drain_record_set(wq3d_fan* f, int s) {
+    wq3d_fan::RecSet& r = f->rs[s];
+    HIP_TRY(rt::event_sync(r.done));
+    unsigned long long cnt[3];
     int32_t err = 0;
-    HIP_TRY(rt::memcpy(cnt, f->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    HIP_TRY(rt::memcpy(cnt, r.count, sizeof(cnt), hipMemcpyDeviceToHost));
     HIP_TRY(rt::memcpy(&err, f->error, sizeof(err), hipMemcpyDeviceToHost));
     if (err & 1) return fail(WQ3D_ERR_CAPACITY, "record buffer overflow");
     if (err & 2) return fail(WQ3D_ERR_CAPACITY, "eigenray candidate buffer overflow");
@@ -811,23 +847,58 @@ wq3d_status collect_records(wq3d_fan* f) {
         f->h_rays.resize(n0 + cnt[0]);
         f->h_ray_int.resize((n0 + cnt[0]) * nf);
         f->h_ray_phase.resize((n0 + cnt[0]) * nf);
-        HIP_TRY(rt::memcpy(f->h_rays.data() + n0, f->R.rays, cnt[0] * sizeof(wq3d_eigenray), hipMemcpyDeviceToHost));
-        HIP_TRY(rt::memcpy(f->h_ray_int.data() + n0 * nf, f->R.ray_int, cnt[0] * nf * sizeof(double), hipMemcpyDeviceToHost));
-        HIP_TRY(rt::memcpy(f->h_ray_phase.data() + n0 * nf, f->R.ray_phase, cnt[0] * nf * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(rt::memcpy(f->h_rays.data() + n0, r.rays, cnt[0] * sizeof(wq3d_eigenray), hipMemcpyDeviceToHost));
+        HIP_TRY(rt::memcpy(f->h_ray_int.data() + n0 * nf, r.ray_int, cnt[0] * nf * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(rt::memcpy(f->h_ray_phase.data() + n0 * nf, r.ray_phase, cnt[0] * nf * sizeof(double), hipMemcpyDeviceToHost));
     }
     if (cnt[1]) {
         const size_t n0 = f->h_verbs.size();
         f->h_verbs.resize(n0 + cnt[1]);
         f->h_verb_power.resize((n0 + cnt[1]) * nf);
-        HIP_TRY(rt::memcpy(f->h_verbs.data() + n0, f->R.verbs, cnt[1] * sizeof(wq3d_eigenverb), hipMemcpyDeviceToHost));
-        HIP_TRY(rt::memcpy(f->h_verb_power.data() + n0 * nf, f->R.verb_power, cnt[1] * nf * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(rt::memcpy(f->h_verbs.data() + n0, r.verbs, cnt[1] * sizeof(wq3d_eigenverb), hipMemcpyDeviceToHost));
+        HIP_TRY(rt::memcpy(f->h_verb_power.data() + n0 * nf, r.verb_power, cnt[1] * nf * sizeof(double), hipMemcpyDeviceToHost));
     }
     if (cnt[2]) {
         const size_t n0 = f->h_refl.size();
         f->h_refl.resize(n0 + cnt[2]);
-        HIP_TRY(rt::memcpy(f->h_refl.data() + n0, f->R.refl, cnt[2] * sizeof(wq3d_reflection), hipMemcpyDeviceToHost));
+        HIP_TRY(rt::memcpy(f->h_refl.data() + n0, r.refl, cnt[2] * sizeof(wq3d_reflection), hipMemcpyDeviceToHost));
     }
-    HIP_TRY(rt::memset_async(f->counters, 0, 3 * sizeof(unsigned long long), f->stream));
+    HIP_TRY(rt::memset_async(r.count, 0, 3 * sizeof(unsigned long long), f->stream));
+    r.pending = false;
+    return WQ3D_OK;
+}
+
+// end of a 64-step batch: mark the active set done, switch the kernels to
+// the other set, and copy that one out first if it still holds the batch
+// before (finished long ago, so this host work overlaps the kernels queued
+// since)
+wq3d_status rotate_record_sets(wq3d_fan* f) {
+    wq3d_fan::RecSet& cur = f->rs[f->rset];
+    HIP_TRY(rt::event_record(cur.done, f->stream));
+    cur.pending = true;
+    const int next = f->rset ^ 1;
+    if (f->rs[next].pending) {
+        wq3d_status st = drain_record_set(f, next);
+        if (st) return st;
+    }
+    use_record_set(f, next);
+    f->pending_steps = 0;
+    return WQ3D_OK;
+}
+
+// every record written so far, in step order, to the host vectors
+wq3d_status collect_records(wq3d_fan* f) {
+    HIP_TRY(rt::sync(f->stream));
+    const int older = f->rset ^ 1;
+    if (f->rs[older].pending) {
+        wq3d_status st = drain_record_set(f, older);
+        if (st) return st;
+    }
+    wq3d_fan::RecSet& cur = f->rs[f->rset];
+    HIP_TRY(rt::event_record(cur.done, f->stream));
+    cur.pending = true;
+    wq3d_status st = drain_record_set(f, f->rset);
+    if (st) return st;
     HIP_TRY(rt::sync(f->stream));
     f->pending_steps = 0;
     return WQ3D_OK;
@@ -1173,19 +1244,22 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         R.verb_cap = std::max<int64_t>(1 << 16, std::min<int64_t>(N * 4, 1 << 22));
         R.refl_cap = std::max<int64_t>(1 << 16, std::min<int64_t>(N * 4, 1 << 22));
         R.cand_cap = nt > 0 ? std::max<int64_t>(1 << 20, std::min<int64_t>(nt * N / 64, 1 << 24)) : 1;
-        if (f->mem.alloc(&R.rays, R.ray_cap) != hipSuccess ||
-            f->mem.alloc(&R.ray_int, R.ray_cap * F.nf) != hipSuccess ||
-            f->mem.alloc(&R.ray_phase, R.ray_cap * F.nf) != hipSuccess ||
-            f->mem.alloc(&R.verbs, R.verb_cap) != hipSuccess ||
-            f->mem.alloc(&R.verb_power, R.verb_cap * F.nf) != hipSuccess ||
-            f->mem.alloc(&R.refl, R.refl_cap) != hipSuccess ||
-            f->mem.alloc(&R.cand, R.cand_cap) != hipSuccess ||
+        for (auto& r : f->rs) {
+            if (f->mem.alloc(&r.rays, R.ray_cap) != hipSuccess ||
+                f->mem.alloc(&r.ray_int, R.ray_cap * F.nf) != hipSuccess ||
+                f->mem.alloc(&r.ray_phase, R.ray_cap * F.nf) != hipSuccess ||
+                f->mem.alloc(&r.verbs, R.verb_cap) != hipSuccess ||
+                f->mem.alloc(&r.verb_power, R.verb_cap * F.nf) != hipSuccess ||
+                f->mem.alloc(&r.refl, R.refl_cap) != hipSuccess || f->mem.alloc(&r.count, 3) != hipSuccess ||
+                rt::memset(r.count, 0, 3 * sizeof(unsigned long long)) != hipSuccess ||
+                rt::event_create(&r.done) != hipSuccess)
+                return bail(fail(WQ3D_ERR_DEVICE, "record allocation failed"));
+        }
+        if (f->mem.alloc(&R.cand, R.cand_cap) != hipSuccess ||
             f->mem.alloc(&f->counters, 6 + 2 * kScanSlots) != hipSuccess || f->mem.alloc(&f->error, 1) != hipSuccess ||
-            f->mem.alloc(&f->kp, 1) != hipSuccess)
+            f->mem.alloc(&f->kp, 2) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "record allocation failed"));
-        R.ray_count = f->counters + 0;
-        R.verb_count = f->counters + 1;
-        R.refl_count = f->counters + 2;
+        use_record_set(f, 0);
         R.cand_count = f->counters + 3;
         R.hit_count = f->counters + 4;
         R.pass_count = f->counters + 5;
@@ -1218,6 +1292,8 @@ wq3d_status wq3d_fan_destroy(wq3d_fan* f) {
         (void)rt::event_destroy(ev.second.second);
     }
     for (hipEvent_t e : f->event_pool) (void)rt::event_destroy(e);
+    for (auto& r : f->rs)
+        if (r.done) (void)rt::event_destroy(r.done);
     (void)rt::stream_destroy(f->stream);
     delete f;
     return WQ3D_OK;
@@ -1231,10 +1307,15 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
     const unsigned grid = (unsigned)((F.N + kBlock - 1) / kBlock);
     const DEnv& E = f->env->dev;
     if (f->kp_dirty) {
-        // after the launches that read the previous copy
+        // after the launches that read the previous copies
         HIP_TRY(rt::sync(f->stream));
-        const KParams kp{E, F, f->A, f->R};
-        HIP_TRY(rt::memcpy(f->kp, &kp, sizeof(KParams), hipMemcpyHostToDevice));
+        const int active = f->rset;
+        for (int r = 0; r < 2; ++r) {
+            use_record_set(f, r);
+            const KParams kp{E, F, f->A, f->R};
+            HIP_TRY(rt::memcpy(f->kp + r, &kp, sizeof(KParams), hipMemcpyHostToDevice));
+        }
+        use_record_set(f, active);
         f->kp_dirty = false;
     }
     // the detect grid: 4 ray tiles (wavefronts) per workgroup times the
@@ -1251,7 +1332,7 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
                 launch_reflect(f->stream, grid, F, f->A, S, f->R);
             }));
             HIP_TRY(timed(f, KID_BOUNCE, [&] {
-                launch_bounce(f->stream, grid, f->kp, E, F, f->A, S, f->R, t, key);
+                launch_bounce(f->stream, grid, f->kp + f->rset, E, F, f->A, S, f->R, t, key);
             }));
         }
         // rotate: past <- prev <- curr <- next <- past
@@ -1276,12 +1357,12 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
             const double t = f->time;
             const int64_t key = f->steps;
             HIP_TRY(timed(f, KID_BUILD, [&] {
-                launch_build(f->stream, f->kp, E, F, f->A, S, f->R, t, key);
+                launch_build(f->stream, f->kp + f->rset, E, F, f->A, S, f->R, t, key);
             }));
         }
         f->pending_steps += 1;
         if (f->pending_steps >= 64) {
-            wq3d_status st = collect_records(f);
+            wq3d_status st = rotate_record_sets(f);
             if (st) return st;
         }
     }
