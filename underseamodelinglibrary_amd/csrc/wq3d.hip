@@ -35,6 +35,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/wq3d.h"
@@ -70,25 +71,17 @@ struct KParams {
 
 #ifndef WQ3D_EMULATE
 // The per-ray kernels come in variants with the environment's kinds fixed
-// at compile time: PK the profile kind (-1: any), FLAT both boundaries
-// flat.  The kernel's own copy of the descriptor is given the constant
-// kinds, so the branches of the other kinds (the grid interpolation above
-// all) fold away, which shrinks the register footprint of the reflection
-// path.  launch_* picks the variant from the descriptor.
+// at compile time (Kinds<PK, FLAT>: PK the profile kind, FLAT both
+// boundaries flat; PK = -1 reads the kinds from the descriptor), so the
+// other kinds' code, the grid interpolation above all, is not compiled into
+// them.  launch_* picks the variant from the descriptor.
 template <int PK, bool FLAT>
-__device__ __forceinline__ void fix_kinds(DEnv& E) {
-    if (PK >= 0) E.profile.kind = PK;
-    if (FLAT) {
-        E.surface.kind = WQ3D_BOUNDARY_FLAT;
-        E.bottom.kind = WQ3D_BOUNDARY_FLAT;
-    }
-}
+using KindsOf = typename std::conditional<PK < 0, AnyKinds, Kinds<PK, FLAT>>::type;
 
 template <int PK, bool FLAT>
 __global__ void __launch_bounds__(kBlock) k_init(DEnv E, FanConst F, Arrays A, Slots S) {
-    fix_kinds<PK, FLAT>(E);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < F.N) body_init(i, E, F, A, S);
+    if (i < F.N) body_init<KindsOf<PK, FLAT>>(i, E, F, A, S);
 }
 
 __global__ void __launch_bounds__(kBlock) k_reflect(FanConst F, Arrays A, Slots S, Records Rc) {
@@ -104,20 +97,20 @@ __global__ void __launch_bounds__(kBlock) k_reflect(FanConst F, Arrays A, Slots 
 
 // the reflecting rays (flag[] != 0; volume scattering needs every ray);
 // rows of the fan reflect together, so most wavefronts leave after one load
+template <int PK, bool FLAT>
 __global__ void __launch_bounds__(kBlock) k_bounce(const KParams* __restrict__ P, Slots S, double time,
                                                   int64_t step_key) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P->F.N) return;
     if (P->A.flag[i] == 0 && !(P->F.want_verbs && P->E.nvol > 0)) return;
-    body_bounce(i, P->E, P->F, P->A, S, P->R, time, step_key);
+    body_bounce<KindsOf<PK, FLAT>>(i, P->E, P->F, P->A, S, P->R, time, step_key);
 }
 
 template <int PK, bool FLAT>
 __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A, Slots S) {
-    fix_kinds<PK, FLAT>(E);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < F.N) {
-        body_advance(i, E, F, A, S);
+        body_advance<KindsOf<PK, FLAT>>(i, E, F, A, S);
         store_window(F, A, S, i);
     }
 }
@@ -386,7 +379,7 @@ inline hipError_t last_error() { return hipSuccess; }
 
 // kernel launches (serial loops over the same bodies in the emulation build)
 #ifndef WQ3D_EMULATE
-// the compile-time environment variant for E (see fix_kinds): 0 generic,
+// the compile-time environment variant for E (see KindsOf): 0 generic,
 // 1 Munk profile with flat boundaries, 2 linear profile with flat boundaries
 int env_variant(const DEnv& E) {
     const bool flat = E.surface.kind == WQ3D_BOUNDARY_FLAT && E.bottom.kind == WQ3D_BOUNDARY_FLAT;
@@ -433,7 +426,16 @@ void launch_reflect(hipStream_t st, unsigned grid, const FanConst& F, const Arra
 void launch_bounce(hipStream_t st, unsigned grid, const KParams* kp, const DEnv& E, const FanConst& F,
                    const Arrays& A, const Slots& S, const Records& R, double t, int64_t key) {
 #ifndef WQ3D_EMULATE
-    hipLaunchKernelGGL(k_bounce, dim3(grid), dim3(kBlock), 0, st, kp, S, t, key);
+    switch (env_variant(E)) {
+    case 1:
+        hipLaunchKernelGGL((k_bounce<WQ3D_PROFILE_MUNK, true>), dim3(grid), dim3(kBlock), 0, st, kp, S, t, key);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_bounce<WQ3D_PROFILE_LINEAR, true>), dim3(grid), dim3(kBlock), 0, st, kp, S, t, key);
+        break;
+    default:
+        hipLaunchKernelGGL((k_bounce<-1, false>), dim3(grid), dim3(kBlock), 0, st, kp, S, t, key);
+    }
 #else
     (void)kp;
     for (int64_t i = 0; i < F.N; ++i) body_bounce(i, E, F, A, S, R, t, key);

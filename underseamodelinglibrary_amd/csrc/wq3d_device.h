@@ -315,18 +315,35 @@ __host__ __device__ double grid_interpolate(const DGrid& g, const double locatio
     return grid_interp<ND - 1>(g, idx, loc, dresult, dv, want);
 }
 
+// Environment kinds known at compile time.  The functions on the
+// propagation and reflection path take a policy K: with AnyKinds they read
+// the kinds from the descriptor; with Kinds<PK, FLAT> the profile kind is PK
+// and both boundaries are flat, so the other kinds' code (the grid
+// interpolation above all) is not compiled in.
+struct AnyKinds {
+    static constexpr int kProfile = -1;
+    static constexpr bool kFlat = false;
+};
+template <int PK, bool FLAT>
+struct Kinds {
+    static constexpr int kProfile = PK;
+    static constexpr bool kFlat = FLAT;
+};
+
 // ------------------------------------------------------------- profiles
 // profile_munk.cc:19-32, profile_linear.cc:14-40, profile_grid.h:68-130,
 // profile_model::adjust_speed (profile_model.cc:14-26)
+template <class K = AnyKinds>
 __host__ __device__ __forceinline__ void sound_speed(const DProfile& P, double rho, double theta, double phi,
                                    double R, double& c, double g[3], bool want_gradient) {
     double grho = 0.0, gtheta = 0.0, gphi = 0.0, speed;
-    if (P.kind == 1) {
+    const int kind = K::kProfile >= 0 ? K::kProfile : P.kind;
+    if (kind == 1) {
         const double alt = rho - R;
         const double z = 2 * (-alt - P.p[0]) / P.p[1];
         speed = (((z - 1.0) + wqm::exp(-z)) * P.p[3] + 1.0) * P.p[2];
         grho = (1.0 - wqm::exp(-z)) * (-P.p[3] * P.p[2] * 2 / P.p[1]);
-    } else if (P.kind == 0) {
+    } else if (kind == 0) {
         const double z = -(rho - R);
         if (z < P.p[2]) {
             speed = P.p[0] + P.p[1] * z;
@@ -372,9 +389,10 @@ __host__ __device__ __forceinline__ double attenuation(const DProfile& P, double
 
 // ------------------------------------------------------------ boundaries
 // boundary_flat.cc:55-63, boundary_grid.h:145-196
+template <class K = AnyKinds>
 __host__ __device__ __forceinline__ void boundary_height(const DBoundary& B, double theta, double phi, double& h,
                                        double n[3], bool want_normal) {
-    if (B.kind == 0) {
+    if (K::kFlat || B.kind == 0) {
         h = B.rho;
         n[0] = B.normal_rho;
         n[1] = 0.0;
@@ -466,9 +484,10 @@ struct Ray {
 };
 
 // wave_front::update minus attenuation (wave_front.cc:125-178,241-250)
+template <class K = AnyKinds>
 __host__ __device__ __forceinline__ void update_ray(const DEnv& E, double R, Ray& w) {
     double g[3];
-    sound_speed(E.profile, w.pr, w.pt, w.pp, R, w.c, g, true);
+    sound_speed<K>(E.profile, w.pr, w.pt, w.pp, R, w.c, g, true);
     w.cr = g[0];
     w.ct = g[1];
     w.cp = g[2];

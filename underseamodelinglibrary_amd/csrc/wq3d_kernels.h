@@ -173,6 +173,7 @@ WQ_HD_INLINE Ray fresh_ray() {
 
 // reflection kind decided from the state detect_reflections() will see
 // (wave_queue.cc:222-256): 1 surface, 2 bottom, 0 none.
+template <class K = AnyKinds>
 WQ_HD_INLINE uint8_t reflect_kind(const DEnv& E, const FanConst& F, double next_r,
                                                 double next_t, double next_p, double cnr, double cnt,
                                                 double cnp) {
@@ -181,13 +182,14 @@ WQ_HD_INLINE uint8_t reflect_kind(const DEnv& E, const FanConst& F, double next_
         if (grazing > 0.0) return 1;
     }
     double h, n[3];
-    boundary_height(E.bottom, next_t, next_p, h, n, false);
+    boundary_height<K>(E.bottom, next_t, next_p, h, n, false);
     return (h - next_r > 0.0) ? 2 : 0;
 }
 
 // ------------------------------------------------------------------ init
 // wave_front::init_wave + update + wave_queue::init_wavefronts
 // (wave_front.cc:84-121, wave_queue.cc:83-85, 114-150), one thread per ray.
+template <class K = AnyKinds>
 WQ_HD void body_init(int64_t i, const DEnv& E, const FanConst& F, const Arrays& A, const Slots& S) {
     const int de = (int)(i / F.naz), az = (int)(i % F.naz);
     const double dt = F.dt;
@@ -201,22 +203,22 @@ WQ_HD void body_init(int64_t i, const DEnv& E, const FanConst& F, const Arrays& 
     cur.nr = cur.nr / F.c0;
     cur.nt = cur.nt / F.c0;
     cur.np = cur.np / F.c0;
-    update_ray(E, F.R, cur);
+    update_ray<K>(E, F.R, cur);
     Ray nxt = fresh_ray(), pst = fresh_ray(), prv = fresh_ray();
     rk1(-dt, cur, nxt);
-    update_ray(E, F.R, nxt);
+    update_ray<K>(E, F.R, nxt);
     rk2(-dt, cur, nxt, pst);
-    update_ray(E, F.R, pst);
+    update_ray<K>(E, F.R, pst);
     rk3(-dt, cur, nxt, pst, prv);
-    update_ray(E, F.R, prv);
+    update_ray<K>(E, F.R, prv);
     rk1(-dt, prv, nxt);
-    update_ray(E, F.R, nxt);
+    update_ray<K>(E, F.R, nxt);
     rk2(-dt, prv, nxt, pst);
-    update_ray(E, F.R, pst);
+    update_ray<K>(E, F.R, pst);
     rk3(-dt, prv, nxt, pst, pst);
-    update_ray(E, F.R, pst);
+    update_ray<K>(E, F.R, pst);
     ab3(dt, pst, prv, cur, nxt);
-    update_ray(E, F.R, nxt);
+    update_ray<K>(E, F.R, nxt);
     const int sl[4] = {S.past, S.prev, S.curr, S.next};
     const Ray* st[4] = {&pst, &prv, &cur, &nxt};
     const int64_t N = F.N;
@@ -233,7 +235,7 @@ WQ_HD void body_init(int64_t i, const DEnv& E, const FanConst& F, const Arrays& 
         }
     }
     A.path[at(S.next, N, i)] = nxt.dist + 0.0;
-    A.flag[i] = reflect_kind(E, F, nxt.pr, nxt.pt, nxt.pp, cur.nr, cur.nt, cur.np);
+    A.flag[i] = reflect_kind<K>(E, F, nxt.pr, nxt.pt, nxt.pp, cur.nr, cur.nt, cur.np);
 }
 
 // ------------------------------------------------------------- reflect
@@ -399,6 +401,7 @@ WQ_HD void volume_scattering(RayCtx& x) {
 // temporaries in registers; the reference's third block advances a
 // temporary it never updates, so its last RK stage reuses the gradient at
 // the collision point (reflection_model.cc:313-318).
+template <class K = AnyKinds>
 WQ_HD void reflection_reinit(RayCtx& x, double time_water, const double pos[3],
                                   const double nd[3]) {
     const DEnv& E = x.E;
@@ -412,34 +415,35 @@ WQ_HD void reflection_reinit(RayCtx& x, double time_water, const double pos[3],
     temp.nr = nd[0];
     temp.nt = nd[1];
     temp.np = nd[2];
-    update_ray(E, R, temp);
+    update_ray<K>(E, R, temp);
     Ray tn = fresh_ray(), tp = fresh_ray(), tc = fresh_ray(), tq = fresh_ray();
     rk1(-time_water, temp, tn);
-    update_ray(E, R, tn);
+    update_ray<K>(E, R, tn);
     rk2(-time_water, temp, tn, tp);
-    update_ray(E, R, tp);
+    update_ray<K>(E, R, tp);
     rk3(-time_water, temp, tn, tp, tc);
-    update_ray(E, R, tc);
+    update_ray<K>(E, R, tc);
     copy_ray(x.A, at(x.S.curr, N, x.i), tc);
     rk1(-dt, tc, tn);
-    update_ray(E, R, tn);
+    update_ray<K>(E, R, tn);
     rk2(-dt, tc, tn, tp);
-    update_ray(E, R, tp);
+    update_ray<K>(E, R, tp);
     rk3(-dt, tc, tn, tp, tq);
-    update_ray(E, R, tq);
+    update_ray<K>(E, R, tq);
     copy_ray(x.A, at(x.S.prev, N, x.i), tq);
     rk1(-dt, tq, tn);
-    update_ray(E, R, tn);
+    update_ray<K>(E, R, tn);
     rk2(-dt, tq, tn, temp);
     rk3(-dt, tq, tn, temp, tp);
-    update_ray(E, R, tp);
+    update_ray<K>(E, R, tp);
     copy_ray(x.A, at(x.S.past, N, x.i), tp);
     ab3(dt, tp, tq, tc, tn);
-    update_ray(E, R, tn);
+    update_ray<K>(E, R, tn);
     copy_ray(x.A, at(x.S.next, N, x.i), tn);
 }
 
 // reflection_model::surface_reflection (reflection_model.cc:187-243)
+template <class K = AnyKinds>
 WQ_HD bool surface_reflection(RayCtx& x) {
     const FanConst& F = x.F;
     const Arrays& A = x.A;
@@ -461,11 +465,12 @@ WQ_HD bool surface_reflection(RayCtx& x) {
         A.phase[kf] -= M_PI;
     }
     nd[0] = -nd[0];
-    reflection_reinit(x, time_water, pos, nd);
+    reflection_reinit<K>(x, time_water, pos, nd);
     return true;
 }
 
 // reflection_model::bottom_reflection (reflection_model.cc:33-182)
+template <class K = AnyKinds>
 WQ_HD bool bottom_reflection(RayCtx& x, double depth) {
     const double MIN_REFLECT = 6.0;
     const FanConst& F = x.F;
@@ -477,7 +482,7 @@ WQ_HD bool bottom_reflection(RayCtx& x, double depth) {
     double c = A.c[kc];
     double c2 = c * c;
     double bottom_rho, nrm[3];
-    boundary_height(x.E.bottom, pos[1], pos[2], bottom_rho, nrm, true);
+    boundary_height<K>(x.E.bottom, pos[1], pos[2], bottom_rho, nrm, true);
     double height_water = pos[0] - bottom_rho;
     const bool shallow = (F.R - bottom_rho) < TOO_SHALLOW;
     if (shallow) {
@@ -495,7 +500,7 @@ WQ_HD bool bottom_reflection(RayCtx& x, double depth) {
     const double dot_water = -height_water * nrm[0];
     const double time_water = smax(0.0, dot_water / dot_full);
     collision_location(x, x.i, time_water, pos, nd, c);
-    boundary_height(x.E.bottom, pos[1], pos[2], bottom_rho, nrm, true);
+    boundary_height<K>(x.E.bottom, pos[1], pos[2], bottom_rho, nrm, true);
     if (shallow) {
         const double Nn = sqrt(nrm[1] * nrm[1] + nrm[2] * nrm[2]);
         nrm[0] = 0.0;
@@ -535,16 +540,17 @@ WQ_HD bool bottom_reflection(RayCtx& x, double depth) {
     nd[0] = nd[0] / Nn;
     nd[1] = nd[1] / Nn;
     nd[2] = nd[2] / Nn;
-    reflection_reinit(x, time_water, pos, nd);
+    reflection_reinit<K>(x, time_water, pos, nd);
     return true;
 }
 
 // detect_reflections_surface / _bottom bodies (wave_queue.cc:222-256)
+template <class K = AnyKinds>
 WQ_HD bool try_surface(RayCtx& x) {
     const int64_t N = x.F.N;
     const int64_t kn = at(x.S.next, N, x.i);
     if (!(x.A.pr[kn] - x.F.R > 0.0)) return false;
-    if (!surface_reflection(x)) return false;
+    if (!surface_reflection<K>(x)) return false;
     const int v = x.A.srf[kn] + 1;
     x.A.srf[kn] = v;
     x.A.srf[at(x.S.curr, N, x.i)] = v;
@@ -554,14 +560,15 @@ WQ_HD bool try_surface(RayCtx& x) {
     return true;
 }
 
+template <class K = AnyKinds>
 WQ_HD bool try_bottom(RayCtx& x) {
     const int64_t N = x.F.N;
     const int64_t kn = at(x.S.next, N, x.i);
     double height, n[3];
-    boundary_height(x.E.bottom, x.A.pt[kn], x.A.pp[kn], height, n, false);
+    boundary_height<K>(x.E.bottom, x.A.pt[kn], x.A.pp[kn], height, n, false);
     const double depth = height - x.A.pr[kn];
     if (!(depth > 0.0)) return false;
-    if (!bottom_reflection(x, depth)) return false;
+    if (!bottom_reflection<K>(x, depth)) return false;
     const int v = x.A.btm[kn] + 1;
     x.A.btm[kn] = v;
     x.A.btm[at(x.S.curr, N, x.i)] = v;
@@ -608,6 +615,7 @@ WQ_HD void body_reflect_light(int64_t i, const FanConst& F, const Arrays& A, con
     }
 }
 
+template <class K = AnyKinds>
 WQ_HD void body_bounce(int64_t i, const DEnv& E, const FanConst& F, const Arrays& A, const Slots& S,
                        const Records& Rc, double time, int64_t step_key) {
     const bool reflects = A.flag[i] != 0;
@@ -618,14 +626,14 @@ WQ_HD void body_bounce(int64_t i, const DEnv& E, const FanConst& F, const Arrays
     if (!reflects) return;
     // surface <-> bottom recursion as a bounded loop
     bool bottom_next;
-    if (try_surface(x)) {
+    if (try_surface<K>(x)) {
         bottom_next = true;
     } else {
-        try_bottom(x);
+        try_bottom<K>(x);
         bottom_next = false;
     }
     for (int n = 0; n < kMaxBounceChain; ++n) {
-        const bool hit = bottom_next ? try_bottom(x) : try_surface(x);
+        const bool hit = bottom_next ? try_bottom<K>(x) : try_surface<K>(x);
         if (!hit) return;
         bottom_next = !bottom_next;
     }
@@ -644,6 +652,7 @@ WQ_HD_INLINE bool is_extremum(const double* r, int64_t base, int naz, int d, int
     return (x < xp && x < xm) || (x > xp && x > xm);
 }
 
+template <class K = AnyKinds>
 WQ_HD void body_advance(int64_t i, const DEnv& E, const FanConst& F, const Arrays& A, const Slots& S) {
     const int64_t N = F.N;
     const int de = (int)(i / F.naz), az = (int)(i % F.naz);
@@ -675,7 +684,7 @@ WQ_HD void body_advance(int64_t i, const DEnv& E, const FanConst& F, const Array
     y1.hr = A.hr[k1]; y1.ht = A.ht[k1]; y1.hp = A.hp[k1];
     y2 = load_ray(A, k2);
     ab3(F.dt, y0, y1, y2, y3);
-    update_ray(E, F.R, y3);
+    update_ray<K>(E, F.R, y3);
     store_updated(A, k3, y3);
     A.path[k3] = y3.dist + A.path[k2];
     for (int f = 0; f < F.nf; ++f) {
@@ -689,7 +698,7 @@ WQ_HD void body_advance(int64_t i, const DEnv& E, const FanConst& F, const Array
     A.upr[k3] = A.upr[k2];
     A.lwr[k3] = A.lwr[k2];
     A.cst[k3] = A.cst[k2];
-    A.flag[i] = reflect_kind(E, F, y3.pr, y3.pt, y3.pp, y2.nr, y2.nt, y2.np);
+    A.flag[i] = reflect_kind<K>(E, F, y3.pr, y3.pt, y3.pp, y2.nr, y2.nt, y2.np);
 }
 
 // ---------------------------------------------------------------- detect
