@@ -23,6 +23,8 @@
  *                          (waveq3d/wave_queue.h:181-216)
  *   wq3d_env_create    ocean_model construction (ocean/ocean_model.h:62-70)
  *                      lowered to the supported analytic / gridded models
+ *   wq3d_sum_eigenrays eigenray_collection::add_eigenray + sum_eigenrays
+ *                          (eigenrays/eigenray_collection.cc:90-191)
  */
 #ifndef WQ3D_H
 #define WQ3D_H
@@ -271,6 +273,33 @@ wq3d_status wq3d_drain_eigenverbs(wq3d_fan* fan, wq3d_eigenverb* out, double* po
                                   int64_t capacity, int64_t* n);
 wq3d_status wq3d_drain_reflections(wq3d_fan* fan, wq3d_reflection* out, int64_t capacity,
                                    int64_t* n);
+
+/* ------------------------------------------------------ eigenray totals */
+/* eigenray_collection's per-target total (eigenrays/eigenray_collection.cc:
+ * 105-191): the eigenray_model of the sum, whose intensity and phase
+ * (num_freq each) are returned in separate arrays, plus what add_eigenray
+ * keeps per target (:90-101).  Targets with no eigenray get the reference's
+ * values for an empty list: 300 dB, phase 0, NaN weighted averages, source
+ * and target AZ 90 degrees, counters -1, initial time 0. */
+typedef struct wq3d_eigenray_total {
+    double travel_time, source_de, source_az, target_de, target_az;
+    double initial_time;  /* eigenray_collection::initial_time(t1, t2) */
+    int32_t surface, bottom, caustic, upper, lower;
+    int32_t count;        /* eigenray_collection::eigenrays(t1, t2).size() */
+} wq3d_eigenray_total;
+
+/* eigenray_collection::sum_eigenrays on the GPU (the current device) for
+ * eigenrays given in the order the collection received them, which is the
+ * order wq3d_drain_eigenrays returns them in.  rays[k].t1 / .t2 index a
+ * num_t1 x num_t2 target grid; intensity / phase hold num_freq values per
+ * ray.  coherent: phasor sum with phase 2 pi f t + phi (the reference's
+ * default), else a sum of pressure amplitudes.  Outputs are num_t1 x num_t2,
+ * row-major: totals, and total_intensity / total_phase with num_freq values
+ * per target.  WQ3D_ERR_ARGUMENT for a ray outside the grid. */
+wq3d_status wq3d_sum_eigenrays(const wq3d_eigenray* rays, const double* intensity, const double* phase,
+                               int64_t num_rays, const double* frequencies, int32_t num_freq,
+                               int64_t num_t1, int64_t num_t2, int32_t coherent,
+                               wq3d_eigenray_total* totals, double* total_intensity, double* total_phase);
 
 /* Copy one queue slot to the host (synchronises the fan's stream). */
 wq3d_status wq3d_snapshot(wq3d_fan* fan, int32_t slot, wq3d_front_view* view);

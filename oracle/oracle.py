@@ -50,6 +50,9 @@ def load():
     lib.orc_take_reflections.argtypes = [vp, C.POINTER(_abi.ReflectionRec)]
     lib.orc_snapshot.argtypes = [vp, C.c_int32, C.POINTER(_abi.FrontView)]
     lib.orc_math.argtypes = [C.c_int32, dp, dp, dp, i64]
+    lib.orc_sum_eigenrays.restype = C.c_int32
+    lib.orc_sum_eigenrays.argtypes = [C.c_void_p, dp, dp, i64, dp, C.c_int32, i64, i64, C.c_int32,
+                                      C.POINTER(_abi.EigenrayTotal), dp, dp]
     lib.orc_grid_interp.restype = C.c_double
     lib.orc_grid_interp.argtypes = [C.POINTER(_abi.Grid), dp, dp]
     lib.orc_sound_speed.argtypes = [C.POINTER(_abi.ProfileDesc), C.c_double, C.c_double,

@@ -20,7 +20,7 @@
  *   - listeners are replaced by record lists with explicit keys.
  *
  * Pinning (what checks this file: tests/test_oracle_known_answers.py,
- * tests/test_oracle_pins.py, tests/test_math.py):
+ * tests/test_oracle_pins.py, tests/test_math.py, tests/test_collection.py):
  *   the reference's known answers for eigenray_basic, eigenray_concave and
  *   eigenray_branch_pt (travel times, angles, intensities, counts),
  *   reflect_flat_test bounce times, refraction_munk_range cycle ranges and
@@ -30,7 +30,9 @@
  *   quadratic / derivative accuracy and exact trilinear reproduction; the
  *   Rayleigh loss against the textbook wavenumber form of the reflection
  *   coefficient and its critical-angle and normal-incidence limits; the
- *   shared elementary functions against glibc in ulps.  Parity unpinned:
+ *   shared elementary functions against glibc in ulps; the eigenray
+ *   collection's 191.4 dB total (eigenrays/test/eigenrays_test.cc:57-88)
+ *   and closed forms of the coherent / incoherent sums.  Parity unpinned:
  *   the unstable 2x2 / diagonal branches of compute_offsets, the hybrid
  *   Gaussian walk termination details, pchip end-point slopes on
  *   non-uniform axes, reflection_reinit numerics beyond bounce times and
@@ -1922,6 +1924,88 @@ void orc_snapshot(void* h, int32_t slot, wq3d_front_view* view) {
 
 /* Shared elementary functions applied element-wise (accuracy tests):
  * fn 0 sin, 1 cos, 2 exp, 3 log, 4 log10, 5 atan2(x, y), 6 asin, 7 10^x. */
+/* eigenray_collection (eigenrays/eigenray_collection.cc): add_eigenray
+ * (:90-101) appends each ray to its target's list in arrival order and keeps
+ * the initial time; sum_eigenrays (:105-191) then sums each list.  Same
+ * arguments and outputs as wq3d_sum_eigenrays. */
+int32_t orc_sum_eigenrays(const wq3d_eigenray* rays, const double* intensity, const double* phase,
+                          int64_t num_rays, const double* freq, int32_t nf, int64_t num_t1, int64_t num_t2,
+                          int32_t coherent, wq3d_eigenray_total* totals, double* total_intensity,
+                          double* total_phase) {
+    using namespace orc;
+    std::vector<std::vector<int64_t>> lists((size_t)(num_t1 * num_t2));
+    std::vector<double> initial((size_t)(num_t1 * num_t2), 0.0);
+    for (int64_t k = 0; k < num_rays; ++k) {  // add_eigenray
+        const int64_t t1 = rays[k].t1, t2 = rays[k].t2;
+        if (t1 < 0 || t1 >= num_t1 || t2 < 0 || t2 >= num_t2) return 1;
+        const size_t t = (size_t)(t1 * num_t2 + t2);
+        lists[t].push_back(k);
+        const double old_initial = initial[t];
+        const double new_initial = rays[k].travel_time;
+        if (old_initial <= 0.0 || old_initial > new_initial) initial[t] = new_initial;
+    }
+    for (int64_t t1 = 0; t1 < num_t1; ++t1) {  // sum_eigenrays
+        for (int64_t t2 = 0; t2 < num_t2; ++t2) {
+            const size_t t = (size_t)(t1 * num_t2 + t2);
+            const std::vector<int64_t>& ray_list = lists[t];
+            double time = 0.0, source_de = 0.0, source_az_x = 0.0, source_az_y = 0.0;
+            double target_de = 0.0, target_az_x = 0.0, target_az_y = 0.0;
+            int surface = -1, bottom = -1, caustic = -1, upper = -1, lower = -1;
+            double wgt = 0.0, max_a = 0.0;
+            for (int f = 0; f < nf; ++f) {
+                double ph_re = 0.0, ph_im = 0.0;
+                for (int64_t r : ray_list) {
+                    const wq3d_eigenray& ray = rays[r];
+                    double a = wqm::pow10(intensity[r * nf + f] / -20.0);
+                    if (coherent) {
+                        double p = kTwoPi * freq[f] * ray.travel_time + phase[r * nf + f];
+                        p = std::fmod(p, kTwoPi);
+                        const double vre = a * wqm::cos(p), vim = a * wqm::sin(p);
+                        ph_re += vre;
+                        ph_im += vim;
+                    } else {
+                        ph_re += a;
+                    }
+                    a *= a;
+                    wgt += a;
+                    time += a * ray.travel_time;
+                    source_de += a * ray.source_de;
+                    source_az_x += a * wqm::sin(to_radians(ray.source_az));
+                    source_az_y += a * wqm::cos(to_radians(ray.source_az));
+                    target_de += a * ray.target_de;
+                    target_az_x += a * wqm::sin(to_radians(ray.target_az));
+                    target_az_y += a * wqm::cos(to_radians(ray.target_az));
+                    if (a > max_a) {
+                        max_a = a;
+                        surface = ray.surface;
+                        bottom = ray.bottom;
+                        caustic = ray.caustic;
+                        upper = ray.upper;
+                        lower = ray.lower;
+                    }
+                }
+                const wqm::cplx phasor = {ph_re, ph_im};
+                total_intensity[t * nf + f] = -20.0 * wqm::log10(smax(1e-15, wqm::cabs(phasor)));
+                total_phase[t * nf + f] = wqm::carg(phasor);
+            }
+            wq3d_eigenray_total& T = totals[t];
+            T.travel_time = time / wgt;
+            T.source_de = source_de / wgt;
+            T.source_az = 90.0 - to_degrees(wqm::atan2(source_az_y, source_az_x));
+            T.target_de = target_de / wgt;
+            T.target_az = 90.0 - to_degrees(wqm::atan2(target_az_y, target_az_x));
+            T.initial_time = initial[t];
+            T.surface = surface;
+            T.bottom = bottom;
+            T.caustic = caustic;
+            T.upper = upper;
+            T.lower = lower;
+            T.count = (int32_t)ray_list.size();
+        }
+    }
+    return 0;
+}
+
 void orc_math(int32_t fn, const double* x, const double* y, double* out, int64_t n) {
     for (int64_t i = 0; i < n; ++i) {
         switch (fn) {

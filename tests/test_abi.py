@@ -39,7 +39,7 @@ STRUCTS = {
     "wq3d_env_desc": _abi.EnvDesc, "wq3d_thresholds": _abi.Thresholds,
     "wq3d_fan_desc": _abi.FanDesc, "wq3d_eigenray": _abi.EigenrayRec,
     "wq3d_eigenverb": _abi.EigenverbRec, "wq3d_reflection": _abi.ReflectionRec,
-    "wq3d_front_view": _abi.FrontView,
+    "wq3d_front_view": _abi.FrontView, "wq3d_eigenray_total": _abi.EigenrayTotal,
 }
 
 
@@ -75,3 +75,7 @@ def test_calls_fail_cleanly_without_device():
     n = C.c_int64()
     assert lib.wq3d_step(None, 1) == _abi.WQ3D_ERR_ARGUMENT
     assert lib.wq3d_drain_eigenrays(None, None, None, None, 0, C.byref(n)) == _abi.WQ3D_ERR_ARGUMENT
+    freq = (C.c_double * 1)(1000.0)
+    tot = (_abi.EigenrayTotal * 1)()
+    ti, tp = (C.c_double * 1)(), (C.c_double * 1)()
+    assert lib.wq3d_sum_eigenrays(None, None, None, 0, freq, 1, 1, 1, 1, tot, ti, tp) == _abi.WQ3D_ERR_DEVICE

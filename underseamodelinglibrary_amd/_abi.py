@@ -85,6 +85,14 @@ class EigenrayRec(C.Structure):
                 ("upper", C.c_int32), ("lower", C.c_int32), ("pad_", C.c_int32)]
 
 
+class EigenrayTotal(C.Structure):
+    _fields_ = [("travel_time", C.c_double), ("source_de", C.c_double),
+                ("source_az", C.c_double), ("target_de", C.c_double),
+                ("target_az", C.c_double), ("initial_time", C.c_double),
+                ("surface", C.c_int32), ("bottom", C.c_int32), ("caustic", C.c_int32),
+                ("upper", C.c_int32), ("lower", C.c_int32), ("count", C.c_int32)]
+
+
 class EigenverbRec(C.Structure):
     _fields_ = [("step", C.c_int64), ("de", C.c_int32), ("az", C.c_int32),
                 ("seq", C.c_int32), ("type", C.c_int32),

@@ -284,6 +284,12 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
     }
 }
 
+// eigenray_collection::sum_eigenrays, one thread per target
+__global__ void __launch_bounds__(kBlock) k_sum_eigenrays(SumIn in, SumOut out, int64_t targets) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < targets) body_sum_eigenrays(t, in, out);
+}
+
 // is_closest_ray + build_eigenray per candidate, grid-stride over the
 // device-side candidate count; each candidate is built by a group of G
 // consecutive lanes, one per frequency (body_build<true>)
@@ -1481,6 +1487,68 @@ wq3d_status wq3d_drain_eigenverbs(wq3d_fan* f, wq3d_eigenverb* out, double* powe
     }
     f->h_verbs.clear();
     f->h_verb_power.clear();
+    return WQ3D_OK;
+}
+
+wq3d_status wq3d_sum_eigenrays(const wq3d_eigenray* rays, const double* intensity, const double* phase,
+                               int64_t num_rays, const double* frequencies, int32_t num_freq,
+                               int64_t num_t1, int64_t num_t2, int32_t coherent,
+                               wq3d_eigenray_total* totals, double* total_intensity, double* total_phase) {
+    if (num_rays < 0 || num_t1 < 0 || num_t2 < 0 || num_freq < 1)
+        return fail(WQ3D_ERR_ARGUMENT, "negative size or no frequency");
+    if ((num_rays > 0 && (!rays || !intensity || !phase)) || !frequencies ||
+        (num_t1 * num_t2 > 0 && (!totals || !total_intensity || !total_phase)))
+        return fail(WQ3D_ERR_ARGUMENT, "null argument");
+    const int64_t targets = num_t1 * num_t2;
+    if (targets == 0) return WQ3D_OK;
+    if (wq3d_device_count() < 1) return fail(WQ3D_ERR_DEVICE, "no HIP device");
+    // the collection's per-target lists: ray indices grouped by target,
+    // each group in arrival order (a stable counting sort)
+    std::vector<int64_t> offset((size_t)targets + 1, 0), order((size_t)std::max<int64_t>(1, num_rays));
+    for (int64_t k = 0; k < num_rays; ++k) {
+        const int64_t t1 = rays[k].t1, t2 = rays[k].t2;
+        if (t1 < 0 || t1 >= num_t1 || t2 < 0 || t2 >= num_t2)
+            return fail(WQ3D_ERR_ARGUMENT, "eigenray target outside the grid");
+        offset[(size_t)(t1 * num_t2 + t2) + 1] += 1;
+    }
+    for (int64_t t = 0; t < targets; ++t) offset[(size_t)t + 1] += offset[(size_t)t];
+    {
+        std::vector<int64_t> fill(offset.begin(), offset.end() - 1);
+        for (int64_t k = 0; k < num_rays; ++k)
+            order[(size_t)fill[(size_t)(rays[k].t1 * num_t2 + rays[k].t2)]++] = k;
+    }
+    DeviceBuffers mem;
+    SumIn in{};
+    SumOut out{};
+    in.nf = num_freq;
+    in.coherent = coherent != 0;
+    const size_t nr = (size_t)std::max<int64_t>(1, num_rays);
+    wq3d_eigenray_total* d_tot = nullptr;
+    double *d_inten = nullptr, *d_phase = nullptr;
+    if (mem.upload(&in.rays, rays ? rays : nullptr, (size_t)num_rays) != hipSuccess ||
+        mem.upload(&in.inten, intensity, (size_t)num_rays * num_freq) != hipSuccess ||
+        mem.upload(&in.phase, phase, (size_t)num_rays * num_freq) != hipSuccess ||
+        mem.upload(&in.order, order.data(), nr) != hipSuccess ||
+        mem.upload(&in.offset, offset.data(), offset.size()) != hipSuccess ||
+        mem.upload(&in.freq, frequencies, (size_t)num_freq) != hipSuccess ||
+        mem.alloc(&d_tot, (size_t)targets) != hipSuccess ||
+        mem.alloc(&d_inten, (size_t)targets * num_freq) != hipSuccess ||
+        mem.alloc(&d_phase, (size_t)targets * num_freq) != hipSuccess)
+        return fail(WQ3D_ERR_DEVICE, "eigenray sum: device allocation or upload failed");
+    out.tot = d_tot;
+    out.inten = d_inten;
+    out.phase = d_phase;
+#ifndef WQ3D_EMULATE
+    hipLaunchKernelGGL(k_sum_eigenrays, dim3((unsigned)((targets + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0,
+                       in, out, targets);
+    HIP_TRY(rt::last_error());
+    HIP_TRY(hipDeviceSynchronize());
+#else
+    for (int64_t t = 0; t < targets; ++t) body_sum_eigenrays(t, in, out);
+#endif
+    HIP_TRY(rt::memcpy(totals, d_tot, (size_t)targets * sizeof(wq3d_eigenray_total), hipMemcpyDeviceToHost));
+    HIP_TRY(rt::memcpy(total_intensity, d_inten, (size_t)targets * num_freq * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(rt::memcpy(total_phase, d_phase, (size_t)targets * num_freq * sizeof(double), hipMemcpyDeviceToHost));
     return WQ3D_OK;
 }
 

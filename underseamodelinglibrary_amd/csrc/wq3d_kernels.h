@@ -640,6 +640,89 @@ WQ_HD void body_bounce(int64_t i, const DEnv& E, const FanConst& F, const Arrays
     atom_or(Rc.error, 4);
 }
 
+// ------------------------------------------------------ eigenray totals
+// eigenray_collection::add_eigenray (eigenrays/eigenray_collection.cc:90-101)
+// and sum_eigenrays (:105-191) for one target, over its eigenrays in the
+// order the collection received them (order[begin .. end)): frequency-major
+// loops, sums in that order, so the result is the reference's bit for bit
+// given the same elementary functions.
+struct SumIn {
+    const wq3d_eigenray* rays;
+    const double *inten, *phase;  // [ray][nf]
+    const int64_t* order;         // ray indices grouped by target, in order
+    const int64_t* offset;        // [targets + 1] into order
+    const double* freq;
+    int32_t nf, coherent;
+};
+
+struct SumOut {
+    wq3d_eigenray_total* tot;
+    double *inten, *phase;  // [target][nf]
+};
+
+WQ_HD void body_sum_eigenrays(int64_t target, const SumIn& in, const SumOut& out) {
+    const double TWO_PI = 2.0 * M_PI;
+    const int64_t b = in.offset[target], e = in.offset[target + 1];
+    const int nf = in.nf;
+    double initial = 0.0;
+    for (int64_t k = b; k < e; ++k) {
+        const double t = in.rays[in.order[k]].travel_time;
+        if (initial <= 0.0 || initial > t) initial = t;
+    }
+    double time = 0.0, source_de = 0.0, source_az_x = 0.0, source_az_y = 0.0;
+    double target_de = 0.0, target_az_x = 0.0, target_az_y = 0.0;
+    int32_t surface = -1, bottom = -1, caustic = -1, upper = -1, lower = -1;
+    double wgt = 0.0, max_a = 0.0;
+    for (int f = 0; f < nf; ++f) {
+        wqm::cplx phasor = {0.0, 0.0};
+        for (int64_t k = b; k < e; ++k) {
+            const int64_t r = in.order[k];
+            const wq3d_eigenray& ray = in.rays[r];
+            double a = wqm::pow10(in.inten[r * nf + f] / -20.0);
+            if (in.coherent) {
+                double p = TWO_PI * in.freq[f] * ray.travel_time + in.phase[r * nf + f];
+                p = ::fmod(p, TWO_PI);
+                phasor.re += a * wqm::cos(p);
+                phasor.im += a * wqm::sin(p);
+            } else {
+                phasor.re += a;
+            }
+            a *= a;
+            wgt += a;
+            time += a * ray.travel_time;
+            source_de += a * ray.source_de;
+            source_az_x += a * wqm::sin(to_radians(ray.source_az));
+            source_az_y += a * wqm::cos(to_radians(ray.source_az));
+            target_de += a * ray.target_de;
+            target_az_x += a * wqm::sin(to_radians(ray.target_az));
+            target_az_y += a * wqm::cos(to_radians(ray.target_az));
+            if (a > max_a) {
+                max_a = a;
+                surface = ray.surface;
+                bottom = ray.bottom;
+                caustic = ray.caustic;
+                upper = ray.upper;
+                lower = ray.lower;
+            }
+        }
+        out.inten[target * nf + f] = -20.0 * wqm::log10(smax(1e-15, wqm::cabs(phasor)));
+        out.phase[target * nf + f] = wqm::carg(phasor);
+    }
+    wq3d_eigenray_total& T = out.tot[target];
+    T.travel_time = time / wgt;
+    T.source_de = source_de / wgt;
+    T.source_az = 90.0 - to_degrees(wqm::atan2(source_az_y, source_az_x));
+    T.target_de = target_de / wgt;
+    T.target_az = 90.0 - to_degrees(wqm::atan2(target_az_y, target_az_x));
+    T.initial_time = initial;
+    T.surface = surface;
+    T.bottom = bottom;
+    T.caustic = caustic;
+    T.upper = upper;
+    T.lower = lower;
+    T.count = (int32_t)(e - b);
+}
+
 // --------------------------------------------------------------- advance
 // find_edges (gather form of wave_front.cc:184-216) on the old next slot,
 // then AB3 + update() of the new next slot and the carries of step()
