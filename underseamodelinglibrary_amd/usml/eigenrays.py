@@ -136,6 +136,76 @@ class eigenray_collection:
     def total(self, t1=0, t2=0):
         return self._total[t1][t2]
 
+    def write_netcdf(self, filename, long_name=None):
+        """eigenray_collection.cc:196-356: the same dimensions, variables,
+        attributes and record layout, in a netCDF classic (CDF-2) file.
+        For each target the summed record comes first, then its eigenrays
+        in arrival order; proploss_index / eigenray_index / eigenray_num
+        locate them.  The reference writes sourceID and targetID as uint64,
+        which the classic format lacks: here they are doubles (exact below
+        2^53)."""
+        from . import netcdf3 as nc
+        n1, n2 = self.size1(), self.size2()
+        nf = self._frequencies.size()
+        w = nc.Writer()
+        if long_name is not None:
+            w.atts["long_name"] = long_name
+        w.atts["Conventions"] = "COARDS"
+        rows, cols = w.add_dim("rows", n1), w.add_dim("cols", n2)
+        ray_dim = w.add_dim("eigenrays", self.num_eigenrays() + n1 * n2)
+        freq_dim = w.add_dim("frequencies", nf)
+        records = []  # eigenray_model per record, in file order
+        prop_index = np.zeros((n1, n2), dtype=np.int32)
+        ray_index = np.zeros((n1, n2), dtype=np.int32)
+        ray_num = np.zeros((n1, n2), dtype=np.int32)
+        for t1 in range(n1):
+            for t2 in range(n2):
+                prop_index[t1, t2] = len(records)
+                ray_index[t1, t2] = len(records) + 1
+                ray_num[t1, t2] = len(self._lists[t1][t2])
+                records.append(self._total[t1][t2])
+                records.extend(self._lists[t1][t2])
+        tp = self._target_pos
+        w.add_var("sourceID", nc.NC_DOUBLE, [], [float(self._sourceID)])
+        w.add_var("source_latitude", nc.NC_DOUBLE, [], [self._source_pos.latitude()],
+                  {"units": "degrees_north"})
+        w.add_var("source_longitude", nc.NC_DOUBLE, [], [self._source_pos.longitude()],
+                  {"units": "degrees_east"})
+        w.add_var("source_altitude", nc.NC_DOUBLE, [], [self._source_pos.altitude()],
+                  {"units": "meters", "positive": "up"})
+        w.add_var("targetID", nc.NC_DOUBLE, [rows, cols], self._targetIDs.astype(np.float64))
+        grid = [[(tp.latitude(t1, t2), tp.longitude(t1, t2), tp.altitude(t1, t2))
+                 for t2 in range(n2)] for t1 in range(n1)]
+        grid = np.array(grid, dtype=np.float64).reshape(n1, n2, 3)
+        w.add_var("latitude", nc.NC_DOUBLE, [rows, cols], grid[:, :, 0], {"units": "degrees_north"})
+        w.add_var("longitude", nc.NC_DOUBLE, [rows, cols], grid[:, :, 1], {"units": "degrees_east"})
+        w.add_var("altitude", nc.NC_DOUBLE, [rows, cols], grid[:, :, 2],
+                  {"units": "meters", "positive": "up"})
+        w.add_var("initial_time", nc.NC_DOUBLE, [rows, cols], self._initial, {"units": "seconds"})
+        w.add_var("frequencies", nc.NC_DOUBLE, [freq_dim],
+                  [self._frequencies(k) for k in range(nf)], {"units": "hertz"})
+        w.add_var("proploss_index", nc.NC_INT, [rows, cols], prop_index, {"units": "count"})
+        w.add_var("eigenray_index", nc.NC_INT, [rows, cols], ray_index, {"units": "count"})
+        w.add_var("eigenray_num", nc.NC_INT, [rows, cols], ray_num, {"units": "count"})
+        w.add_var("intensity", nc.NC_DOUBLE, [ray_dim, freq_dim],
+                  [np.asarray(r.intensity, dtype=np.float64) for r in records] or np.zeros((0, nf)),
+                  {"units": "dB"})
+        w.add_var("phase", nc.NC_DOUBLE, [ray_dim, freq_dim],
+                  [np.asarray(r.phase, dtype=np.float64) for r in records] or np.zeros((0, nf)),
+                  {"units": "radians"})
+        w.add_var("travel_time", nc.NC_DOUBLE, [ray_dim], [r.travel_time for r in records],
+                  {"units": "seconds"})
+        angle_atts = {"source_de": {"units": "degrees", "positive": "up"},
+                      "source_az": {"units": "degrees_true", "positive": "clockwise"},
+                      "target_de": {"units": "degrees", "positive": "up"},
+                      "target_az": {"units": "degrees_true", "positive": "clockwise"}}
+        for name, atts in angle_atts.items():
+            w.add_var(name, nc.NC_DOUBLE, [ray_dim], [getattr(r, name) for r in records], atts)
+        for name in ("surface", "bottom", "caustic", "upper", "lower"):
+            w.add_var(name, nc.NC_SHORT, [ray_dim], [getattr(r, name) for r in records],
+                      {"units": "count"})
+        w.write(filename)
+
     def sum_eigenrays(self):
         """eigenray_collection.cc:105-191, on the GPU."""
         nf = self._frequencies.size()
