@@ -53,7 +53,6 @@ constexpr int kBuildLanes = 32;  // lanes per eigenray candidate in k_build
 // records), one ray per lane, and hit keys buffered per wavefront before a
 // flush
 constexpr int kTile = kWinTile;
-constexpr int kSupersPerWave = 4;  // super-blocks of targets per k_detect wavefront
 constexpr int kHitQueue = 128;
 // MI355X accelerator dies (XCDs), each with its own L2; consecutive
 // workgroups of a launch are dispatched to them in turn
@@ -151,10 +150,8 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
     const int wv = threadIdx.x >> 6;
     const int64_t L = blockIdx.x;
     const int64_t k = L / kXcds;
-    const int64_t nsg = (F.nsup + kSupersPerWave - 1) / kSupersPerWave;
-    const int64_t quad = L % kXcds + kXcds * (k / nsg);
-    const int64_t sup0 = (k % nsg) * kSupersPerWave;
-    const int nsp = (int)((F.nsup - sup0 < kSupersPerWave) ? F.nsup - sup0 : kSupersPerWave);
+    const int64_t quad = L % kXcds + kXcds * (k / F.nsup);
+    const int64_t sup = k % F.nsup;
     if (quad >= quads) return;  // whole workgroup, before the one barrier
     // this lane's ray: tile of 8 AZ x 8 DE rays
     int64_t ray = -1;
@@ -165,35 +162,44 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
         const int64_t az = (tile % tiles_az) * kTile + lane % kTile;
         if (de < F.nde && az < F.naz) ray = de * F.naz + az;
     }
-    // the super-blocks' boxes and their blocks' boxes, staged in LDS for all
-    // four wavefronts: [super][block, or kSuperBlock for the super box]
-    __shared__ double s_box[kSupersPerWave][kSuperBlock + 1][8];
-    __shared__ uint32_t s_br[kSupersPerWave][kSuperBlock + 1];
-    for (int e = threadIdx.x; e < kSupersPerWave * (kSuperBlock + 1) * 8; e += kBlock) {
-        const int u = e / ((kSuperBlock + 1) * 8), q = (e / 8) % (kSuperBlock + 1), c = e % 8;
-        const int64_t sup = sup0 + (u < nsp ? u : 0);
-        const int64_t nb_u = (F.nblk - sup * kSuperBlock < kSuperBlock) ? F.nblk - sup * kSuperBlock : kSuperBlock;
-        // slot kSuperBlock (and the unused ones): the super box
-        const int64_t bi = q < nb_u ? sup * kSuperBlock + q : F.nblk + sup;
-        s_box[u][q][c] = F.tgt_box[bi * 8 + c];
-        if (c == 0) s_br[u][q] = F.blk_branch[bi];
+    // the super-block's box and its blocks' boxes, staged in LDS for all
+    // four wavefronts
+    __shared__ double s_box[kSuperBlock + 1][8];
+    __shared__ uint32_t s_br[kSuperBlock + 1];
+    const int64_t b_begin = sup * kSuperBlock;
+    const int nbk = (int)((F.nblk - b_begin < kSuperBlock) ? F.nblk - b_begin : kSuperBlock);
+    if (threadIdx.x < (kSuperBlock + 1) * 8) {
+        const int q = threadIdx.x >> 3, e = threadIdx.x & 7;
+        const int64_t bi = q < nbk ? b_begin + q : F.nblk + sup;  // slot kSuperBlock (and unused ones): the super box
+        s_box[q][e] = F.tgt_box[bi * 8 + e];
+        if (e == 0) s_br[q] = F.blk_branch[bi];
     }
     __syncthreads();
-    auto box_at = [&](int u, int q) {
+    auto box_at = [&](int q) {
         TargetBox B;
-        B.at0 = s_box[u][q][0];
-        B.hx = s_box[u][q][1];
-        B.bt0 = s_box[u][q][2];
-        B.hy = s_box[u][q][3];
-        B.r_lo = s_box[u][q][4];
-        B.r_hi = s_box[u][q][5];
-        B.s_lo = s_box[u][q][6];
-        B.s_hi = s_box[u][q][7];
+        B.at0 = s_box[q][0];
+        B.hx = s_box[q][1];
+        B.bt0 = s_box[q][2];
+        B.hy = s_box[q][3];
+        B.r_lo = s_box[q][4];
+        B.r_hi = s_box[q][5];
+        B.s_lo = s_box[q][6];
+        B.s_hi = s_box[q][7];
         return B;
     };
     DetectCell C;
-    const bool searched = load_cell(F, A, S, ray, C);
-    if (!__any(searched)) return;  // wave-uniform; no barriers below
+    bool active = load_cell(F, A, S, ray, C);
+    // (ray, box) bounds tested: the super-block's for every searched ray,
+    // each block's for the rays the super-block's bound kept
+    unsigned long long boxes_tested = (unsigned long long)__popcll(__ballot(active));
+    active = active && block_cell_ok(C, box_at(kSuperBlock), s_br[kSuperBlock] != 0);
+    boxes_tested += (unsigned long long)__popcll(__ballot(active)) * (unsigned long long)nbk;
+    if (lane == 0 && boxes_tested) atomicAdd(&Rc.scan[(blockIdx.x % kScanSlots) * 2 + 1], boxes_tested);
+    if (!__any(active)) return;  // wave-uniform; no barriers below
+    // the blocks each ray keeps, decided for all blocks before any pair test
+    unsigned long long keep[kSuperBlock];
+    for (int q = 0; q < kSuperBlock; ++q)
+        keep[q] = __ballot(active && q < nbk && block_cell_ok(C, box_at(q), s_br[q] != 0));
     unsigned qn = 0;                // candidate keys in this wavefront's buffer
     unsigned long long tested = 0;  // (target, ray) pairs put to the pair test
     unsigned long long hits = 0;    // pairs the filter kept
@@ -232,53 +238,36 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
         qn += cnt;
     };
     const ConstLinFeatures* cf = (const ConstLinFeatures*)(F.tgt_lin);
-    unsigned long long boxes_tested = 0;
-    for (int u = 0; u < nsp; ++u) {
-        const int64_t b_begin = (sup0 + u) * kSuperBlock;
-        const int nbk = (int)((F.nblk - b_begin < kSuperBlock) ? F.nblk - b_begin : kSuperBlock);
-        // (ray, box) bounds tested: the super-block's for every searched
-        // ray, each block's for the rays the super-block's bound kept
-        boxes_tested += (unsigned long long)__popcll(__ballot(searched));
-        const bool active = searched && block_cell_ok(C, box_at(u, kSuperBlock), s_br[u][kSuperBlock] != 0);
-        const unsigned long long kept_rays = __ballot(active);
-        if (!kept_rays) continue;
-        boxes_tested += (unsigned long long)__popcll(kept_rays) * (unsigned long long)nbk;
-        // the blocks each ray keeps, decided for all blocks before any pair test
-        unsigned long long keep[kSuperBlock];
-        for (int q = 0; q < kSuperBlock; ++q)
-            keep[q] = __ballot(active && q < nbk && block_cell_ok(C, box_at(u, q), s_br[u][q] != 0));
-        for (int q = 0; q < nbk; ++q) {
-            const unsigned long long kq = keep[q];
-            if (!kq) continue;
-            const bool need = (kq >> lane) & 1ull;
-            const int64_t t0 = (b_begin + q) * kTargetBlock;
-            const int64_t t_end = (F.nt < t0 + kTargetBlock) ? F.nt : t0 + kTargetBlock;
-            tested += (unsigned long long)__popcll(kq) * (unsigned long long)(t_end - t0);
-            const bool bbr = s_br[u][q] != 0;
-            // four targets per round: one batch of scalar loads, four window
-            // forms as independent chains
-            int64_t t = t0;
-            for (; t + 4 <= t_end; t += 4) {
-                const LinFeatures f0 = cf[t], f1 = cf[t + 1], f2 = cf[t + 2], f3 = cf[t + 3];
-                const bool i0 = need && detect_window_in(C.W, f0.f);
-                const bool i1 = need && detect_window_in(C.W, f1.f);
-                const bool i2 = need && detect_window_in(C.W, f2.f);
-                const bool i3 = need && detect_window_in(C.W, f3.f);
-                if (!__any(i0 || i1 || i2 || i3)) continue;
-                visit(i0, f0.f, t, bbr);
-                visit(i1, f1.f, t + 1, bbr);
-                visit(i2, f2.f, t + 2, bbr);
-                visit(i3, f3.f, t + 3, bbr);
-            }
-            for (; t < t_end; ++t) {
-                const LinFeatures fc = cf[t];
-                visit(need && detect_window_in(C.W, fc.f), fc.f, t, bbr);
-            }
+    for (int q = 0; q < nbk; ++q) {
+        const unsigned long long kq = keep[q];
+        if (!kq) continue;
+        const bool need = (kq >> lane) & 1ull;
+        const int64_t t0 = (b_begin + q) * kTargetBlock;
+        const int64_t t_end = (F.nt < t0 + kTargetBlock) ? F.nt : t0 + kTargetBlock;
+        tested += (unsigned long long)__popcll(kq) * (unsigned long long)(t_end - t0);
+        const bool bbr = s_br[q] != 0;
+        // four targets per round: one batch of scalar loads, four window
+        // forms as independent chains
+        int64_t t = t0;
+        for (; t + 4 <= t_end; t += 4) {
+            const LinFeatures f0 = cf[t], f1 = cf[t + 1], f2 = cf[t + 2], f3 = cf[t + 3];
+            const bool i0 = need && detect_window_in(C.W, f0.f);
+            const bool i1 = need && detect_window_in(C.W, f1.f);
+            const bool i2 = need && detect_window_in(C.W, f2.f);
+            const bool i3 = need && detect_window_in(C.W, f3.f);
+            if (!__any(i0 || i1 || i2 || i3)) continue;
+            visit(i0, f0.f, t, bbr);
+            visit(i1, f1.f, t + 1, bbr);
+            visit(i2, f2.f, t + 2, bbr);
+            visit(i3, f3.f, t + 3, bbr);
+        }
+        for (; t < t_end; ++t) {
+            const LinFeatures fc = cf[t];
+            visit(need && detect_window_in(C.W, fc.f), fc.f, t, bbr);
         }
     }
     if (qn) flush();
     if (lane == 0) {
-        if (boxes_tested) atomicAdd(&Rc.scan[(blockIdx.x % kScanSlots) * 2 + 1], boxes_tested);
         if (tested) atomicAdd(&Rc.scan[(blockIdx.x % kScanSlots) * 2], tested);
         if (hits) atomicAdd(Rc.hit_count, hits);
     }
@@ -474,7 +463,7 @@ void launch_advance(hipStream_t st, unsigned grid, const DEnv& E, const FanConst
 void launch_detect(hipStream_t st, int64_t quads, const FanConst& F, const Arrays& A, const Slots& S,
                    const Records& R) {
 #ifndef WQ3D_EMULATE
-    const int64_t grid = kXcds * ((quads + kXcds - 1) / kXcds) * ((F.nsup + kSupersPerWave - 1) / kSupersPerWave);
+    const int64_t grid = kXcds * ((quads + kXcds - 1) / kXcds) * F.nsup;
     hipLaunchKernelGGL(k_detect, dim3((unsigned)grid), dim3(kBlock), 0, st, F, A, S, R, quads);
 #else
     const TargetBox* boxes = reinterpret_cast<const TargetBox*>(F.tgt_box);
