@@ -13,7 +13,7 @@ import numpy as np
 
 from underseamodelinglibrary_amd import _abi
 from underseamodelinglibrary_amd.usml.types import wposition1
-from underseamodelinglibrary_amd.usml.waveq3d import queue_base, wave_front
+from underseamodelinglibrary_amd.usml.waveq3d import front_buffers, queue_base, wave_front
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libwq3d_oracle.so")
@@ -82,6 +82,7 @@ class oracle_queue(queue_base):
         self._steps = 0
 
     def __del__(self):
+        self.close_netcdf()
         if getattr(self, "_h", None):
             self._lib.orc_destroy(self._h)
             self._h = None
@@ -138,20 +139,8 @@ class oracle_queue(queue_base):
         self._lib.orc_take_reflections(self._h, recs.ctypes.data_as(C.POINTER(_abi.ReflectionRec)))
         return recs
 
-    def snapshot_arrays(self, slot):
-        nde, naz, nf = self.num_de(), self.num_az(), self._freq.size()
-        n = nde * naz
-        arrays = {}
-        view = _abi.FrontView()
-        for name in _abi.FRONT_FIELDS:
-            size = n * nf if name in ("attenuation", "phase") else n
-            arrays[name] = np.zeros(size)
-            setattr(view, name, arrays[name].ctypes.data_as(C.POINTER(C.c_double)))
-        for name in _abi.FRONT_INT_FIELDS:
-            arrays[name] = np.zeros(n, dtype=np.int32)
-            setattr(view, name, arrays[name].ctypes.data_as(C.POINTER(C.c_int32)))
-        arrays["on_edge"] = np.zeros(n, dtype=np.uint8)
-        view.on_edge = arrays["on_edge"].ctypes.data_as(C.POINTER(C.c_uint8))
+    def snapshot_arrays(self, slot, fields=None):
+        arrays, view = front_buffers(self.num_de() * self.num_az(), self._freq.size(), fields)
         self._lib.orc_snapshot(self._h, slot, C.byref(view))
         return arrays
 

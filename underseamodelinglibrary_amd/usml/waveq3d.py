@@ -93,6 +93,27 @@ class wave_front:
         self.arrays = g
 
 
+def front_buffers(n, nf, fields=None):
+    """Host arrays for the named fields of one queue slot (all of them when
+    fields is None) and the C view over them; fields not named stay NULL in
+    the view and are not copied."""
+    want = None if fields is None else set(fields)
+    arrays = {}
+    view = _abi.FrontView()
+    for name in _abi.FRONT_FIELDS:
+        if want is None or name in want:
+            arrays[name] = np.zeros(n * nf if name in ("attenuation", "phase") else n)
+            setattr(view, name, arrays[name].ctypes.data_as(C.POINTER(C.c_double)))
+    for name in _abi.FRONT_INT_FIELDS:
+        if want is None or name in want:
+            arrays[name] = np.zeros(n, dtype=np.int32)
+            setattr(view, name, arrays[name].ctypes.data_as(C.POINTER(C.c_int32)))
+    if want is None or "on_edge" in want:
+        arrays["on_edge"] = np.zeros(n, dtype=np.uint8)
+        view.on_edge = arrays["on_edge"].ctypes.data_as(C.POINTER(C.c_uint8))
+    return arrays, view
+
+
 def default_thresholds():
     t = _abi.Thresholds()
     t.intensity = 300.0
@@ -314,6 +335,71 @@ class queue_base:
     def _push_listeners(self):
         pass
 
+    # -- wavefront log (waveq3d/wave_queue_netcdf.cc:14-108, CDL in
+    # wave_queue.h:760-814).  A backend provides time() and
+    # snapshot_arrays(slot, fields).
+    _LOG_FIELDS = ("pos_rho", "pos_theta", "pos_phi") + tuple(_abi.FRONT_INT_FIELDS) + ("on_edge",)
+    _LOG_COUNTS = ("surface", "bottom", "caustic", "upper", "lower")
+
+    def init_netcdf(self, filename, long_name=None):
+        """Open the wavefront log (wave_queue_netcdf.cc:14-77): dimensions
+        frequencies, source_de, source_az and the unlimited travel_time;
+        the fan's coordinates; per record the time and the curr() positions
+        as latitude / longitude / altitude, the five path counters (short)
+        and on_edge (byte), indexed (travel_time, source_de, source_az).
+        The file is netCDF classic (64-bit offset)."""
+        from . import netcdf3 as nc
+        self.close_netcdf()
+        nf, nde, naz = self._freq.size(), self.num_de(), self.num_az()
+        w = nc.Writer()
+        if long_name is not None:
+            w.atts["long_name"] = long_name
+        w.atts["Conventions"] = "COARDS"
+        fdim = w.add_dim("frequencies", nf)
+        ddim = w.add_dim("source_de", nde)
+        adim = w.add_dim("source_az", naz)
+        tdim = w.add_dim("travel_time", nc.UNLIMITED)
+        tda = [tdim, ddim, adim]
+        w.add_var("frequencies", nc.NC_DOUBLE, [fdim], [self._freq(k) for k in range(nf)],
+                  {"units": "hertz"})
+        w.add_var("source_de", nc.NC_DOUBLE, [ddim], [self._de(k) for k in range(nde)],
+                  {"units": "degrees", "positive": "up"})
+        w.add_var("source_az", nc.NC_DOUBLE, [adim], [self._az(k) for k in range(naz)],
+                  {"units": "degrees_true", "positive": "clockwise"})
+        w.add_var("travel_time", nc.NC_DOUBLE, [tdim], atts={"units": "seconds"})
+        w.add_var("latitude", nc.NC_DOUBLE, tda, atts={"units": "degrees_north"})
+        w.add_var("longitude", nc.NC_DOUBLE, tda, atts={"units": "degrees_east"})
+        w.add_var("altitude", nc.NC_DOUBLE, tda, atts={"units": "meters", "positive": "up"})
+        for name in self._LOG_COUNTS:
+            w.add_var(name, nc.NC_SHORT, tda, atts={"units": "count"})
+        w.add_var("on_edge", nc.NC_BYTE, tda, atts={"units": "bool"})
+        w.create(filename)
+        self._nc_log = w
+
+    def save_netcdf(self):
+        """Append the current wavefront as one record (wave_queue_netcdf.cc:
+        82-101): wposition's latitude = 90 - theta (180/pi), longitude =
+        phi (180/pi), altitude = rho - earth radius (types/wposition.h)."""
+        w = getattr(self, "_nc_log", None)
+        if w is None:
+            raise RuntimeError("save_netcdf() before init_netcdf()")
+        a = self.snapshot_arrays(_abi.CURR, self._LOG_FIELDS)
+        rec = {"travel_time": self.time(),
+               "latitude": 90.0 - a["pos_theta"] * (180.0 / np.pi),
+               "longitude": a["pos_phi"] * (180.0 / np.pi),
+               "altitude": a["pos_rho"] - earth_radius(),
+               "on_edge": a["on_edge"]}
+        for name in self._LOG_COUNTS:
+            rec[name] = a[name]
+        w.put_record(rec)
+
+    def close_netcdf(self):
+        """wave_queue_netcdf.cc:106-108."""
+        w = getattr(self, "_nc_log", None)
+        if w is not None:
+            w.close()
+            self._nc_log = None
+
 
 def _load_lib():
     from .. import _lib
@@ -343,6 +429,7 @@ class wave_queue(queue_base):
         self._source = wposition1.from_rtp(r.value, t.value, p.value)
 
     def close(self):
+        self.close_netcdf()
         if getattr(self, "_fan", None) and self._fan.value:
             self._lib.wq3d_fan_destroy(self._fan)
             self._fan = C.c_void_p()
@@ -438,20 +525,8 @@ class wave_queue(queue_base):
             self._fan, recs.ctypes.data_as(C.POINTER(_abi.ReflectionRec)), n.value, C.byref(n)))
         return recs[:n.value]
 
-    def snapshot_arrays(self, slot):
-        nde, naz, nf = self.num_de(), self.num_az(), self._freq.size()
-        n = nde * naz
-        arrays = {}
-        view = _abi.FrontView()
-        for name in _abi.FRONT_FIELDS:
-            size = n * nf if name in ("attenuation", "phase") else n
-            arrays[name] = np.zeros(size)
-            setattr(view, name, arrays[name].ctypes.data_as(C.POINTER(C.c_double)))
-        for name in _abi.FRONT_INT_FIELDS:
-            arrays[name] = np.zeros(n, dtype=np.int32)
-            setattr(view, name, arrays[name].ctypes.data_as(C.POINTER(C.c_int32)))
-        arrays["on_edge"] = np.zeros(n, dtype=np.uint8)
-        view.on_edge = arrays["on_edge"].ctypes.data_as(C.POINTER(C.c_uint8))
+    def snapshot_arrays(self, slot, fields=None):
+        arrays, view = front_buffers(self.num_de() * self.num_az(), self._freq.size(), fields)
         self._lib.check(self._lib.wq3d_snapshot(self._fan, slot, C.byref(view)))
         return arrays
 
@@ -473,13 +548,3 @@ class wave_queue(queue_base):
 
     def sync(self):
         self._lib.check(self._lib.wq3d_sync(self._fan))
-
-    # -- netCDF wavefront log (waveq3d/wave_queue_netcdf.cc) is not provided
-    def init_netcdf(self, filename, long_name=None):
-        raise NotImplementedError("wavefront netCDF log is not supported")
-
-    def save_netcdf(self):
-        raise NotImplementedError("wavefront netCDF log is not supported")
-
-    def close_netcdf(self):
-        raise NotImplementedError("wavefront netCDF log is not supported")
