@@ -55,6 +55,49 @@ def cpu_baseline(steps=2):
                       f"targets ({dt:.1f} s)"}
 
 
+def batch_leg(lib, n_sources, chunk=64):
+    """The production caller's form: wavefront_generator tasks for the first
+    n_sources of config 4's seeded sources (uneven fans, 4-16 frequencies,
+    90 s), no targets, in config 1's analytic ocean.  Timed once one after
+    another (each fan alone, as the reference's thread pool would hand them
+    to one device) and once as one wavefront_batch (every fan on its own
+    stream, chunks of steps queued on all fans before any is waited on).
+    Fan creation and release are inside both timings."""
+    from underseamodelinglibrary_amd import scenarios
+    from underseamodelinglibrary_amd.usml.wavegen import (wavefront_batch, wavefront_generator,
+                                                          wavefront_source)
+    ocean = scenarios.config1().ocean
+    specs = scenarios.config4_sources(256)[:n_sources]
+    ray_steps = sum(sp["de"].size() * sp["az"].size() * sp["steps"] for sp in specs)
+
+    def make():
+        return [wavefront_generator(wavefront_source(sp["pos"], keyID=k), None, None, sp["freq"],
+                                    sp["de"], sp["az"], sp["time_step"], sp["time_max"], 300.0,
+                                    999, 999, ocean=ocean)
+                for k, sp in enumerate(specs)]
+
+    log(f"[bench] wavefront_batch: {n_sources} fans, {ray_steps / 1e9:.2f} G ray-steps")
+    gens = make()
+    t0 = time.perf_counter()
+    for g in gens:
+        g.run(chunk=chunk)
+    alone_s = time.perf_counter() - t0
+    assert all(g.done() for g in gens)
+    gens = make()
+    t0 = time.perf_counter()
+    wavefront_batch(gens, chunk=chunk).run()
+    batch_s = time.perf_counter() - t0
+    assert all(g.done() for g in gens)
+    log(f"[bench] wavefront_batch: alone {alone_s:.2f} s, batch {batch_s:.2f} s")
+    return {"workload": f"config 4 sources 1-{n_sources} of 256 (seeded), uneven fans "
+                        "DE 61-721 x AZ 1-181, 4-16 freqs, 90 s, no targets, Munk + flat "
+                        "5000 m bottom (gridded ocean off on GPU)",
+            "fans": n_sources, "ray_steps": ray_steps,
+            "alone_ray_steps_per_s": ray_steps / alone_s,
+            "batch_ray_steps_per_s": ray_steps / batch_s,
+            "speedup": alone_s / batch_s}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +105,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--batch-fans", type=int, default=32,
+                    help="sources of the wavefront_batch leg (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -170,6 +215,8 @@ def main():
         coll["cpu_ms"] = (time.perf_counter() - t0) * 1e3
         coll["cpu_kind"] = "port, 1 core"
     out["eigenray_collection"] = coll
+    if args.batch_fans > 0:
+        out["wavefront_batch"] = batch_leg(lib, args.batch_fans)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)

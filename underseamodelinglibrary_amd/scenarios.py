@@ -223,3 +223,29 @@ def eigenray_basic():
     az = seq_linear(-4.0, 1.0, 4.0)
     target = wposition(1, 1, 45.02, -45.0, -1000.0)
     return scenario("eigenray_basic", ocean, freq, pos, de, az, 0.1, target, loop_steps(0.1, 3.5))
+
+
+def config4_sources(n_sources=256, seed=4096, time_max=90.0):
+    """Config 4's independent sources: seeded positions in a 5 x 5 deg box
+    around 45N 45W at 10-2000 m depth, uneven fans (DE count uniform in
+    61-721 spread over -80..80 deg, AZ count uniform in 1-181 spread evenly
+    round the circle), 4-16 log-spaced frequencies over 100-10000 Hz, dt
+    0.1 s, t < time_max.  Returns a list of dicts with the source position,
+    fans and frequencies; every source shares one ocean, which the caller
+    picks (config 2's gridded one, or config 1's analytic one where grids
+    are unavailable)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_sources):
+        lat = 45.0 + rng.uniform(-2.5, 2.5)
+        lng = -45.0 + rng.uniform(-2.5, 2.5)
+        depth = rng.uniform(10.0, 2000.0)
+        nde = int(rng.integers(61, 722))
+        naz = int(rng.integers(1, 182))
+        nf = int(rng.integers(4, 17))
+        de = seq_linear(-80.0, 160.0 / (nde - 1), nde)
+        az = seq_linear(0.0, 360.0 / naz, naz)
+        freq = seq_log(100.0, 100.0 ** (1.0 / (nf - 1)), nf)
+        out.append({"pos": wposition1(lat, lng, -depth), "de": de, "az": az, "freq": freq,
+                    "time_step": 0.1, "time_max": time_max, "steps": loop_steps(0.1, time_max)})
+    return out

@@ -335,6 +335,13 @@ class queue_base:
     def _push_listeners(self):
         pass
 
+    # a backend whose step() is synchronous completes the steps in step_end
+    def step_begin(self, n=1):
+        pass
+
+    def step_end(self, n=1):
+        self.step(n)
+
     # -- wavefront log (waveq3d/wave_queue_netcdf.cc:14-108, CDL in
     # wave_queue.h:760-814).  A backend provides time() and
     # snapshot_arrays(slot, fields).
@@ -459,8 +466,17 @@ class wave_queue(queue_base):
     def step(self, n=1):
         """One wave_queue::step() per call (n > 1 batches steps on the device
         and replays listeners per step afterwards)."""
-        lib = self._lib
-        lib.check(lib.wq3d_step(self._fan, int(n)))
+        self.step_begin(n)
+        self.step_end(n)
+
+    def step_begin(self, n=1):
+        """Queue n steps on the fan's stream and return at once; step_end(n)
+        completes them.  Fans on different queues overlap on the device."""
+        self._lib.check(self._lib.wq3d_step(self._fan, int(n)))
+
+    def step_end(self, n=1):
+        """Wait for the n steps queued by step_begin(n), advance time() and
+        replay their listener calls step by step."""
         if not (self._ray_listeners or self._verb_listeners or self._refl_listeners):
             for _ in range(int(n)):
                 self._time += self._time_step
