@@ -216,3 +216,55 @@ def test_search_counts_match_emulation():
     assert 0 < h.value and t.value <= sc.num_rays() * sc.num_targets() * sc.steps and b.value > 0
     g._lib.check(g._lib.wq3d_search_totals(g._fan, C.byref(t), C.byref(b)))
     assert t.value == 0 and b.value == 0
+
+
+def test_tiny_and_ragged_fans_parity(oracle_mod):
+    """Fans smaller than one 8 x 8 ray tile and fans whose sizes are not
+    multiples of it (1 x 1, 1 x 5, 2 x 1, 3 x 9, 13 x 7), with targets."""
+    total = 0
+    for de, az, nt in ((seq_linear(-10.0, 1.0, 1), seq_linear(0.0, 1.0, 1), (1, 1)),
+                       (seq_linear(-10.0, 1.0, 1), seq_linear(-2.0, 1.0, 5), (2, 2)),
+                       (seq_linear(-10.0, 20.0, 2), seq_linear(0.0, 1.0, 1), (1, 1)),
+                       (seq_linear(-20.0, 10.0, 3), seq_linear(-4.0, 1.0, 9), (3, 3)),
+                       (seq_linear(-30.0, 5.0, 13), seq_linear(-6.0, 2.0, 7), (4, 4))):
+        sc = scenarios.config1(de=de, az=az, n_targets=nt, time_max=15.0)
+        g, o = _run(sc, oracle_mod)
+        total += _check_all(g, o, sc.freq.size())
+    assert total > 0
+
+
+def test_single_ray_reflection_known_answer(oracle_mod):
+    """waveq3d/test/reflection_test.cc:168-280 on the device: one ray between
+    a flat surface and a 1000 m flat bottom reflects at k * 7.450560973 s
+    (within 1e-4 s) and latitude 45 + 0.1 k (within 1e-6 deg), and the
+    reflection records equal the oracle's."""
+    from underseamodelinglibrary_amd.usml.types import compute_earth_radius, wposition1
+
+    class Events:
+        def __init__(self):
+            self.events = []
+
+        def reflect(self, time, de, az, dt, grazing, speed, position, ndirection, type):
+            self.events.append((time, position.latitude(), type))
+
+    compute_earth_radius(45.0)
+    ocean = oc.ocean_model(oc.boundary_flat(), oc.boundary_flat(1000.0), oc.profile_linear())
+    args = (ocean, seq_log(10.0, 10.0, 1), wposition1(45.0, -45.0, 0.0),
+            seq_linear(-5.183617057, 0.0, 1), seq_linear(0.0, 0.0, 1), 0.1)
+    g = wave_queue(*args)
+    cb = Events()
+    g.add_reflection_listener(cb)
+    while g.time() < 60.0:
+        g.step()
+    assert len(cb.events) >= 7
+    for k, (t, lat, typ) in enumerate(cb.events, start=1):
+        assert abs(t - k * 7.450560973) < 1e-4
+        assert abs(lat - (45.0 + k * 0.1)) < 1e-6
+    o = oracle_mod.oracle_queue(*args)
+    oc_cb = Events()
+    o.add_reflection_listener(oc_cb)
+    while o.time() < 60.0:
+        o.step()
+    assert len(oc_cb.events) == len(cb.events)
+    for (tg, lg, yg), (to, lo, yo) in zip(cb.events, oc_cb.events):
+        assert yg == yo and abs(tg - to) <= 1e-9 * abs(to) and abs(lg - lo) <= 1e-9
