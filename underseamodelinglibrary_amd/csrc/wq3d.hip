@@ -1,15 +1,15 @@
 // wq3d.hip - HIP kernels (gfx950, FP64) and the C ABI of libwq3d.so.
 //
-// One wave_queue::step() (waveq3d/wave_queue.cc:155-192) is five launches on
+// One wave_queue::step() (waveq3d/wave_queue.cc:155-192) is four launches on
 // the fan's stream:
-//   k_reflect  detect_reflections without find_edges (wave_queue.cc:197-292):
-//              one thread per ray for the caustic and vertex counters; the
-//              reference's de-major caustic dependency is a gather on the
-//              lower DE neighbour's pre-reflection state, which is read-only
-//              here because whether that neighbour reflects (flag[]) was
-//              decided by the previous k_advance.
-//   k_bounce   the reflections of the rays flag[] marks
-//              (reflection_model.cc:33-360).
+//   k_bounce   detect_reflections without find_edges (wave_queue.cc:197-292):
+//              one thread per ray for the caustic and vertex counters, then
+//              the reflections of the rays flag[] marks
+//              (reflection_model.cc:33-360).  The reference's de-major
+//              caustic dependency is a gather on the lower DE neighbour's
+//              pre-reflection state, which no thread writes because a ray
+//              reads that neighbour only when it does not reflect (flag[],
+//              decided by the previous k_advance).
 //   k_advance  find_edges on the old next slot (wave_front.cc:184-216, as a
 //              gather), queue rotation, AB3 + update() of the new next slot
 //              (ode_integ.cc:131-186, wave_front.cc:125-178), attenuation and
@@ -84,24 +84,21 @@ __global__ void __launch_bounds__(kBlock) k_init(DEnv E, FanConst F, Arrays A, S
     if (i < F.N) body_init<KindsOf<PK, FLAT>>(i, E, F, A, S);
 }
 
-__global__ void __launch_bounds__(kBlock) k_reflect(FanConst F, Arrays A, Slots S, Records Rc) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // the eigenray search counters of this step (read by later launches)
-    if (i == 0) {
-        *Rc.cand_count = 0ull;
-        *Rc.hit_count = 0ull;
-        *Rc.pass_count = 0ull;
-    }
-    if (i < F.N) body_reflect_light(i, F, A, S);
-}
-
-// the reflecting rays (flag[] != 0; volume scattering needs every ray);
-// rows of the fan reflect together, so most wavefronts leave after one load
+// detect_reflections: every ray's caustic and vertex counters, then the
+// reflecting rays (flag[] != 0; volume scattering needs every ray).  Rows of
+// the fan reflect together, so most wavefronts leave after the light part.
 template <int PK, bool FLAT>
 __global__ void __launch_bounds__(kBlock) k_bounce(const KParams* __restrict__ P, Slots S, double time,
                                                   int64_t step_key) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // the eigenray search counters of this step (read by later launches)
+    if (i == 0) {
+        *P->R.cand_count = 0ull;
+        *P->R.hit_count = 0ull;
+        *P->R.pass_count = 0ull;
+    }
     if (i >= P->F.N) return;
+    body_reflect_light(i, P->F, P->A, S);
     if (P->A.flag[i] == 0 && !(P->F.want_verbs && P->E.nvol > 0)) return;
     body_bounce<KindsOf<PK, FLAT>>(i, P->E, P->F, P->A, S, P->R, time, step_key);
 }
@@ -417,18 +414,6 @@ void launch_init(hipStream_t st, unsigned grid, const DEnv& E, const FanConst& F
 #endif
 }
 
-void launch_reflect(hipStream_t st, unsigned grid, const FanConst& F, const Arrays& A, const Slots& S,
-                    const Records& R) {
-#ifndef WQ3D_EMULATE
-    hipLaunchKernelGGL(k_reflect, dim3(grid), dim3(kBlock), 0, st, F, A, S, R);
-#else
-    *R.cand_count = 0ull;
-    *R.hit_count = 0ull;
-    *R.pass_count = 0ull;
-    for (int64_t i = 0; i < F.N; ++i) body_reflect_light(i, F, A, S);
-#endif
-}
-
 void launch_bounce(hipStream_t st, unsigned grid, const KParams* kp, const DEnv& E, const FanConst& F,
                    const Arrays& A, const Slots& S, const Records& R, double t, int64_t key) {
 #ifndef WQ3D_EMULATE
@@ -444,7 +429,13 @@ void launch_bounce(hipStream_t st, unsigned grid, const KParams* kp, const DEnv&
     }
 #else
     (void)kp;
-    for (int64_t i = 0; i < F.N; ++i) body_bounce(i, E, F, A, S, R, t, key);
+    *R.cand_count = 0ull;
+    *R.hit_count = 0ull;
+    *R.pass_count = 0ull;
+    for (int64_t i = 0; i < F.N; ++i) {
+        body_reflect_light(i, F, A, S);
+        body_bounce(i, E, F, A, S, R, t, key);
+    }
 #endif
 }
 
@@ -795,7 +786,7 @@ hipEvent_t take_event(wq3d_fan* f) {
     return e;
 }
 
-// kernel ids for timing
+// kernel ids for timing (1 and 5 are no longer launched; the ids stay)
 enum { KID_INIT = 0, KID_REFLECT = 1, KID_ADVANCE = 2, KID_DETECT = 3, KID_BUILD = 4, KID_EXACT = 5, KID_BOUNCE = 6,
        KID_COUNT = 7 };
 
@@ -1336,9 +1327,6 @@ wq3d_status wq3d_step(wq3d_fan* f, int64_t n) {
             const Slots S = slots_of(f);
             const double t = f->time;
             const int64_t key = f->steps;
-            HIP_TRY(timed(f, KID_REFLECT, [&] {
-                launch_reflect(f->stream, grid, F, f->A, S, f->R);
-            }));
             HIP_TRY(timed(f, KID_BOUNCE, [&] {
                 launch_bounce(f->stream, grid, f->kp + f->rset, E, F, f->A, S, f->R, t, key);
             }));
