@@ -1,6 +1,7 @@
 // The reference's eigenray_basic scenario (waveq3d/test/eigenray_test.cc:86-198)
 // driven through the C++ wave_queue mirror over libwq3d.so.  Prints one line
-// per listener call; tests/test_cpp_facade.py checks them.
+// per listener call and the eigenray collection's total for the target;
+// tests/test_cpp_facade.py checks them.
 #include <cstdio>
 #include <exception>
 
@@ -32,8 +33,16 @@ int main() {
         wave.runID(7);
         printer p;
         wave.add_eigenray_listener(&p);
+        eigenrays::eigenray_collection collection(freq, pos, target, 11, {42});
+        wave.add_eigenray_listener(&collection);
         while (wave.time() < 3.5) wave.step();
         std::printf("checks %d time %.17g\n", p.checks, wave.time());
+        collection.sum_eigenrays();
+        const auto total = collection.total(0, 0);
+        std::printf("total %zu %.17g %.17g %.17g %.17g %.17g %d %d %.17g %llu\n", collection.num_eigenrays(),
+                    total->time, total->intensity[0], total->intensity[2], total->phase[0], total->source_de,
+                    total->surface, total->bottom, collection.initial_time(0, 0),
+                    (unsigned long long)collection.targetID(0, 0));
         const auto& curr = wave.curr();
         std::printf("curr %zu %zu %.17g\n", curr.num_de, curr.num_az, curr.pos_rho[curr.index(12, 4)]);
         // an unsupported model is rejected with std::invalid_argument

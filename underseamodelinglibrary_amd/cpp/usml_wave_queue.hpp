@@ -685,6 +685,113 @@ private:
 };
 
 }  // namespace waveq3d
+
+// -------------------------------------------------------------- eigenrays
+namespace eigenrays {
+
+// eigenrays/eigenray_collection.h:34-262: an eigenray listener that files
+// each eigenray under its target in arrival order (add_eigenray,
+// eigenray_collection.cc:90-101) and sums each target's list on the GPU
+// (sum_eigenrays, :105-191, through wq3d_sum_eigenrays).
+class eigenray_collection : public waveq3d::eigenray_listener {
+public:
+    eigenray_collection(std::shared_ptr<const types::seq_vector> frequencies, const types::wposition1& source_pos,
+                        const types::wposition& target_pos, uint64_t sourceID = 0,
+                        std::vector<uint64_t> targetIDs = {}, bool coherent = true)
+        : _freq(std::move(frequencies)), _source(source_pos), _n1(target_pos.size1()),
+          _n2(target_pos.size2()), _sourceID(sourceID), _targetIDs(std::move(targetIDs)), _coherent(coherent),
+          _lists(_n1 * _n2), _initial(_n1 * _n2, 0.0), _total(_n1 * _n2) {
+        if (_targetIDs.empty()) _targetIDs.assign(_n1 * _n2, 0);
+        for (auto& t : _total) t = empty_total();
+    }
+
+    void add_eigenray(size_t t1, size_t t2, waveq3d::eigenray_model::csptr ray, size_t) override {
+        const size_t k = t1 * _n2 + t2;
+        _arrivals.emplace_back(t1, t2, ray);
+        _lists[k].push_back(ray);
+        if (_initial[k] <= 0.0 || _initial[k] > ray->time) _initial[k] = ray->time;
+    }
+
+    void sum_eigenrays() {
+        const size_t nf = _freq->size();
+        std::vector<wq3d_eigenray> recs(_arrivals.size());
+        std::vector<double> inten(recs.size() * nf), phase(recs.size() * nf);
+        for (size_t k = 0; k < _arrivals.size(); ++k) {
+            const auto& r = *std::get<2>(_arrivals[k]);
+            wq3d_eigenray& e = recs[k];
+            e = wq3d_eigenray{};
+            e.t1 = (int32_t)std::get<0>(_arrivals[k]);
+            e.t2 = (int32_t)std::get<1>(_arrivals[k]);
+            e.travel_time = r.time;
+            e.source_de = r.source_de;
+            e.source_az = r.source_az;
+            e.target_de = r.target_de;
+            e.target_az = r.target_az;
+            e.surface = r.surface;
+            e.bottom = r.bottom;
+            e.caustic = r.caustic;
+            e.upper = r.upper;
+            e.lower = r.lower;
+            std::copy(r.intensity.begin(), r.intensity.end(), inten.begin() + k * nf);
+            std::copy(r.phase.begin(), r.phase.end(), phase.begin() + k * nf);
+        }
+        const size_t nt = _n1 * _n2;
+        std::vector<wq3d_eigenray_total> tot(nt);
+        std::vector<double> ti(nt * nf), tp(nt * nf);
+        check(wq3d_sum_eigenrays(recs.data(), inten.data(), phase.data(), (int64_t)recs.size(),
+                                 _freq->values().data(), (int32_t)nf, (int64_t)_n1, (int64_t)_n2,
+                                 _coherent ? 1 : 0, tot.data(), ti.data(), tp.data()));
+        for (size_t k = 0; k < nt; ++k) {
+            auto m = std::make_shared<waveq3d::eigenray_model>();
+            m->time = tot[k].travel_time;
+            m->source_de = tot[k].source_de;
+            m->source_az = tot[k].source_az;
+            m->target_de = tot[k].target_de;
+            m->target_az = tot[k].target_az;
+            m->surface = tot[k].surface;
+            m->bottom = tot[k].bottom;
+            m->caustic = tot[k].caustic;
+            m->upper = tot[k].upper;
+            m->lower = tot[k].lower;
+            m->intensity.assign(ti.begin() + k * nf, ti.begin() + (k + 1) * nf);
+            m->phase.assign(tp.begin() + k * nf, tp.begin() + (k + 1) * nf);
+            _total[k] = m;
+        }
+    }
+
+    size_t size1() const { return _n1; }
+    size_t size2() const { return _n2; }
+    uint64_t sourceID() const { return _sourceID; }
+    uint64_t targetID(size_t t1, size_t t2) const { return _targetIDs[t1 * _n2 + t2]; }
+    bool coherent() const { return _coherent; }
+    size_t num_eigenrays() const { return _arrivals.size(); }
+    const std::vector<waveq3d::eigenray_model::csptr>& eigenrays(size_t t1, size_t t2) const {
+        return _lists[t1 * _n2 + t2];
+    }
+    double initial_time(size_t t1, size_t t2) const { return _initial[t1 * _n2 + t2]; }
+    waveq3d::eigenray_model::csptr total(size_t t1, size_t t2) const { return _total[t1 * _n2 + t2]; }
+
+private:
+    waveq3d::eigenray_model::csptr empty_total() const {
+        auto m = std::make_shared<waveq3d::eigenray_model>();
+        m->intensity.assign(_freq->size(), 0.0);
+        m->phase.assign(_freq->size(), 0.0);
+        return m;
+    }
+
+    std::shared_ptr<const types::seq_vector> _freq;
+    types::wposition1 _source;
+    size_t _n1, _n2;
+    uint64_t _sourceID;
+    std::vector<uint64_t> _targetIDs;
+    bool _coherent;
+    std::vector<std::tuple<size_t, size_t, waveq3d::eigenray_model::csptr>> _arrivals;
+    std::vector<std::vector<waveq3d::eigenray_model::csptr>> _lists;
+    std::vector<double> _initial;
+    std::vector<waveq3d::eigenray_model::csptr> _total;
+};
+
+}  // namespace eigenrays
 }  // namespace usml_amd
 
 #endif  // USML_AMD_WAVE_QUEUE_HPP
