@@ -122,18 +122,18 @@ typedef __attribute__((address_space(4))) const LinFeatures ConstLinFeatures;
 typedef const LinFeatures ConstLinFeatures;
 #endif
 
-// Eigenray search, first stage.  A wavefront takes a tile of 8 x 8
-// neighbouring rays and one super-block of targets (kSuperBlock blocks of
-// 64, consecutive on the targets' Morton curve).  Each lane holds one ray's
-// candidate region (DetectCell: the pre-test window and the four neighbour
-// forms) and tests it against the super-block's box, then against each
-// block's box (block_cell_ok).  For the blocks some ray kept, a lane becomes
-// one of the block's 64 targets: the kept rays are visited one at a time,
-// their forms read from the ray's lane (readlane), and every lane tests its
-// target (detect_cell_ok, in stages so that later forms are read only while
-// some target is still in).  The few pairs that pass are buffered per
-// wavefront in LDS and written to the global hit list with one atomic per
-// flush.
+// Eigenray search filter.  A wavefront takes a tile of 8 x 8 neighbouring
+// rays and one super-block of targets (kSuperBlock blocks of 64,
+// consecutive on the targets' Morton curve).  Each lane holds one ray's
+// candidate region (DetectCell: the pre-test window, the previous-time form
+// and the four neighbour forms) and tests it against the super-block's box,
+// then against each block's box (block_cell_ok); the keep masks of all
+// blocks are decided before any pair test.  For each block some ray kept,
+// the block's targets are read four at a time with scalar loads (the same
+// target for every lane) and every lane whose ray kept the block tests them:
+// the window first, the later forms only while some lane is still in
+// (visit).  The few pairs that pass are buffered per wavefront in LDS and
+// appended to the candidate list with one atomic per flush.
 //
 // The grid is one-dimensional and XCD-aware: workgroups go round-robin to
 // the kXcds dies, so workgroup L runs on die L % kXcds.  Each die gets every

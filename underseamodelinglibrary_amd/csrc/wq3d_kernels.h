@@ -786,13 +786,15 @@ WQ_HD void body_advance(int64_t i, const DEnv& E, const FanConst& F, const Array
 
 // ---------------------------------------------------------------- detect
 // The eigenray search of detect_eigenrays (wave_queue.cc:372-398) is split
-// in three: k_detect applies a window test equivalent to the time-CPA
-// pre-test's outcome (see detect_window) to every (target, interior ray)
-// pair; k_exact decides the pairs inside the window with the exact pre-test
-// and the current-time part of is_closest_ray for the four in-plane
-// neighbours; k_build runs is_closest_ray in full on the survivors.  A pair
-// is only ever dropped on a condition for which is_closest_ray would return
-// false.
+// in two: k_detect filters every (target, interior ray) pair with linear
+// forms in the target's features - a window equivalent to the time-CPA
+// pre-test's outcome (see detect_window), the previous-time comparison and
+// the current-time comparison with the four in-plane neighbours, each with
+// a rounding margin so that it never drops a pair the exact test keeps -
+// and k_build runs the exact pre-test and is_closest_ray in full on the
+// pairs it keeps.  A pair is only ever dropped on a condition for which
+// is_closest_ray would return false.  detect_pair below is the exact test
+// of one pair, which the emulation build's filter check compares against.
 //
 // Distance: the expression of wave_front.cc:227-232 with per-ray and
 // per-target terms hoisted.  It is the same IEEE value: 0.5*(a-b) equals
@@ -1031,16 +1033,6 @@ WQ_HD_INLINE bool block_cell_ok(const DetectCell& C, const TargetBox& B, bool br
     return true;
 }
 
-// Ray points of the exact stage, read from the state arrays on demand.
-struct ArrayPoints {
-    const Arrays& A;
-    const Slots& S;
-    int64_t N;
-    WQ_HD_INLINE DPoint nxt(int64_t i) const { return dpoint(A, at(S.next, N, i)); }
-    WQ_HD_INLINE DPoint prv(int64_t i) const { return dpoint(A, at(S.prev, N, i)); }
-    WQ_HD_INLINE DPoint nb(int64_t j) const { return dpoint(A, at(S.curr, N, j)); }
-};
-
 // The rays detect_eigenrays visits: interior and not on an edge.
 WQ_HD_INLINE bool detect_interior(const FanConst& F, const Arrays& A, const Slots& S, int64_t i) {
     if (i < 0 || i >= F.N) return false;
@@ -1148,22 +1140,6 @@ WQ_HD_INLINE bool load_cell(const FanConst& F, const Arrays& A, const Slots& S, 
         for (int j = 0; j < 6; ++j) C.nb[n].a[j] = in[(17 + 7 * n + j) * kWinField];
     }
     C.use = cell_use(F, A, S, i, (uint32_t)in[15 * kWinField] - 1u);
-    return true;
-}
-
-// Ray data of the exact stage: current point and the in-plane neighbours.
-WQ_HD_INLINE bool detect_ray(const FanConst& F, const Arrays& A, const Slots& S, int64_t i,
-                             DetectRay& r) {
-    if (!detect_interior(F, A, S, i)) return false;
-    const int64_t N = F.N;
-    r.i = i;
-    r.cur = dpoint(A, at(S.curr, N, i));
-    uint32_t stat;
-    ray_neighbours(F, i, r.nbj, stat, r.upper);
-    // the DE neighbours (n = 0, 1) are skipped when on an edge
-    r.use = stat;
-    for (int n = 0; n < 2; ++n)
-        if (((stat >> n) & 1u) && A.edge[at(S.curr, N, r.nbj[n])] != 0) r.use &= ~(1u << n);
     return true;
 }
 
