@@ -112,15 +112,11 @@ __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A
     }
 }
 
+
+// one target's six features (tgt_lin rows)
 struct alignas(16) LinFeatures {
     double f[6];
 };
-#if defined(__HIP_DEVICE_COMPILE__)
-// constant address space: wave-uniform reads become scalar loads
-typedef __attribute__((address_space(4))) const LinFeatures ConstLinFeatures;
-#else
-typedef const LinFeatures ConstLinFeatures;
-#endif
 
 // Eigenray search filter.  A wavefront takes a tile of 8 x 8 neighbouring
 // rays and one super-block of targets (kSuperBlock blocks of 64,
@@ -129,10 +125,10 @@ typedef const LinFeatures ConstLinFeatures;
 // and the four neighbour forms) and tests it against the super-block's box,
 // then against each block's box (block_cell_ok); the keep masks of all
 // blocks are decided before any pair test.  For each block some ray kept,
-// the block's targets are read four at a time with scalar loads (the same
-// target for every lane) and every lane whose ray kept the block tests them:
-// the window first, the later forms only while some lane is still in
-// (visit).  The few pairs that pass are buffered per wavefront in LDS and
+// the wavefront copies the block's 64 target feature rows to LDS (one row
+// per lane), then reads them three at a time (the same target for every
+// lane) and every lane whose ray kept the block tests them: the window
+// first, the later forms only while some lane is still in (visit).  The few pairs that pass are buffered per wavefront in LDS and
 // appended to the candidate list with one atomic per flush.
 //
 // The grid is one-dimensional and XCD-aware: workgroups go round-robin to
@@ -234,7 +230,10 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
                 ((unsigned long long)F.tgt_perm[t] << 32) | (unsigned long long)ray;
         qn += cnt;
     };
-    const ConstLinFeatures* cf = (const ConstLinFeatures*)(F.tgt_lin);
+    // this wavefront's copy of the block of targets being tested: one
+    // vector load per lane instead of a chain of scalar loads per target
+    __shared__ LinFeatures s_tf[kBlock / 64][kTargetBlock];
+    const LinFeatures* lf = s_tf[wv];
     for (int q = 0; q < nbk; ++q) {
         const unsigned long long kq = keep[q];
         if (!kq) continue;
@@ -243,23 +242,24 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
         const int64_t t_end = (F.nt < t0 + kTargetBlock) ? F.nt : t0 + kTargetBlock;
         tested += (unsigned long long)__popcll(kq) * (unsigned long long)(t_end - t0);
         const bool bbr = s_br[q] != 0;
-        // four targets per round: one batch of scalar loads, four window
-        // forms as independent chains
+        wave_sync();  // the previous block's reads come first
+        if (t0 + lane < t_end) s_tf[wv][lane] = reinterpret_cast<const LinFeatures*>(F.tgt_lin)[t0 + lane];
+        wave_sync();
+        // three targets per round (the same for every lane, read from LDS),
+        // three window forms as independent chains
         int64_t t = t0;
-        for (; t + 4 <= t_end; t += 4) {
-            const LinFeatures f0 = cf[t], f1 = cf[t + 1], f2 = cf[t + 2], f3 = cf[t + 3];
+        for (; t + 3 <= t_end; t += 3) {
+            const LinFeatures f0 = lf[t - t0], f1 = lf[t + 1 - t0], f2 = lf[t + 2 - t0];
             const bool i0 = need && detect_window_in(C.W, f0.f);
             const bool i1 = need && detect_window_in(C.W, f1.f);
             const bool i2 = need && detect_window_in(C.W, f2.f);
-            const bool i3 = need && detect_window_in(C.W, f3.f);
-            if (!__any(i0 || i1 || i2 || i3)) continue;
+            if (!__any(i0 || i1 || i2)) continue;
             visit(i0, f0.f, t, bbr);
             visit(i1, f1.f, t + 1, bbr);
             visit(i2, f2.f, t + 2, bbr);
-            visit(i3, f3.f, t + 3, bbr);
         }
         for (; t < t_end; ++t) {
-            const LinFeatures fc = cf[t];
+            const LinFeatures fc = lf[t - t0];
             visit(need && detect_window_in(C.W, fc.f), fc.f, t, bbr);
         }
     }
