@@ -1060,7 +1060,7 @@ WQ_HD_INLINE bool detect_ray_window(const FanConst& F, const Arrays& A, const Sl
 // prev.k, prev.a[6], 1 + stat bits (0: ray not searched), nb[n].k,
 // nb[n].a[6] for n = 0..3}.  The neighbours' points are the curr slot's,
 // which k_advance does not write; their edge flags are written by the same
-// launch, so the reader applies them (cell_use).
+// launch, so the reader applies them (load_cell).
 constexpr int kWinStride = 44;
 // Records are stored tile by tile (kWinTile x kWinTile rays, AZ fastest,
 // the k_detect wavefront's tile) and field-major within a tile, so that a
@@ -1109,37 +1109,30 @@ WQ_HD_INLINE void store_window(const FanConst& F, const Arrays& A, const Slots& 
     }
 }
 
-// use bits of a stored record: the AZ rule's bits, less the DE neighbours
-// on an edge of the curr slot
-WQ_HD_INLINE uint32_t cell_use(const FanConst& F, const Arrays& A, const Slots& S, int64_t i, uint32_t stat) {
-    uint32_t use = stat;
-    if ((use & 1u) && A.edge[at(S.curr, F.N, i - F.naz)] != 0) use &= ~1u;
-    if ((use & 2u) && A.edge[at(S.curr, F.N, i + F.naz)] != 0) use &= ~2u;
-    return use;
-}
-
-WQ_HD_INLINE bool load_window(const FanConst& F, const Arrays& A, int64_t i, DetectWindow& W) {
+// the whole record of ray i as a DetectCell; false for rays not searched.
+// Every field and the DE neighbours' edge flags are read before the
+// searched flag is looked at, so the reads overlap in one round trip.
+WQ_HD_INLINE bool load_cell(const FanConst& F, const Arrays& A, const Slots& S, int64_t i, DetectCell& C) {
     if (i < 0 || i >= F.N) return false;
     const double* in = A.win + win_slot(F, i);
-    if (in[15 * kWinField] == 0.0) return false;
-    W.form.k = in[0];
-    for (int j = 0; j < 6; ++j) W.form.a[j] = in[(1 + j) * kWinField];
-    W.w = in[7 * kWinField];
-    W.prev.k = in[8 * kWinField];
-    for (int j = 0; j < 6; ++j) W.prev.a[j] = in[(9 + j) * kWinField];
-    W.m = F.lin_margin;
-    return true;
-}
-
-// the whole record of ray i as a DetectCell; false for rays not searched
-WQ_HD_INLINE bool load_cell(const FanConst& F, const Arrays& A, const Slots& S, int64_t i, DetectCell& C) {
-    if (!load_window(F, A, i, C.W)) return false;
-    const double* in = A.win + win_slot(F, i);
+    const double searched = in[15 * kWinField];
+    C.W.form.k = in[0];
+    for (int j = 0; j < 6; ++j) C.W.form.a[j] = in[(1 + j) * kWinField];
+    C.W.w = in[7 * kWinField];
+    C.W.prev.k = in[8 * kWinField];
+    for (int j = 0; j < 6; ++j) C.W.prev.a[j] = in[(9 + j) * kWinField];
+    C.W.m = F.lin_margin;
     for (int n = 0; n < 4; ++n) {
         C.nb[n].k = in[(16 + 7 * n) * kWinField];
         for (int j = 0; j < 6; ++j) C.nb[n].a[j] = in[(17 + 7 * n + j) * kWinField];
     }
-    C.use = cell_use(F, A, S, i, (uint32_t)in[15 * kWinField] - 1u);
+    const bool edge_lo = i - F.naz >= 0 && A.edge[at(S.curr, F.N, i - F.naz)] != 0;
+    const bool edge_hi = i + F.naz < F.N && A.edge[at(S.curr, F.N, i + F.naz)] != 0;
+    if (searched == 0.0) return false;
+    uint32_t use = (uint32_t)searched - 1u;
+    if ((use & 1u) && edge_lo) use &= ~1u;
+    if ((use & 2u) && edge_hi) use &= ~2u;
+    C.use = use;
     return true;
 }
 
