@@ -38,7 +38,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(steps=2):
+def cpu_baseline(steps=3):
     """Oracle (scalar C++ port, 1 thread) on `steps` steps of the same fan,
     environment and targets."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -104,7 +104,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1181)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--batch-fans", type=int, default=32,
                     help="sources of the wavefront_batch leg (0 skips it)")
     args = ap.parse_args()
