@@ -180,12 +180,21 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
         B.s_hi = s_box[q][7];
         return B;
     };
+    // the super-block's bound: the window part on the window fields of the
+    // records, then the rest of the records for wavefronts still in
     DetectCell C;
-    bool active = load_cell(F, A, S, ray, C);
+    bool active = load_cell_window(F, A, ray, C);
     // (ray, box) bounds tested: the super-block's for every searched ray,
     // each block's for the rays the super-block's bound kept
     unsigned long long boxes_tested = (unsigned long long)__popcll(__ballot(active));
-    active = active && block_cell_ok(C, box_at(kSuperBlock), s_br[kSuperBlock] != 0);
+    {
+        const TargetBox SB = box_at(kSuperBlock);
+        active = active && block_window_ok(C.W, SB);
+        if (__any(active)) {
+            load_cell_rest(F, A, S, ray, C);
+            active = active && (s_br[kSuperBlock] != 0 || block_cell_nb_ok(C, SB));
+        }
+    }
     boxes_tested += (unsigned long long)__popcll(__ballot(active)) * (unsigned long long)nbk;
     if (lane == 0 && boxes_tested) atomicAdd(&Rc.scan[(blockIdx.x % kScanSlots) * 2 + 1], boxes_tested);
     if (!__any(active)) return;  // wave-uniform; no barriers below

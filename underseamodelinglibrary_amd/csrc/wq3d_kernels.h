@@ -1021,9 +1021,8 @@ WQ_HD_INLINE bool detect_cell_ok(const DetectCell& C, const double f[6], bool br
 // pass (blocks holding a source-branch target skip the neighbour part).
 // The neighbour bound gets one more margin for its own rounding, as the
 // window bound does.
-WQ_HD_INLINE bool block_cell_ok(const DetectCell& C, const TargetBox& B, bool branch) {
-    if (!block_window_ok(C.W, B)) return false;
-    if (branch) return true;
+// the neighbour part of block_cell_ok
+WQ_HD_INLINE bool block_cell_nb_ok(const DetectCell& C, const TargetBox& B) {
     for (int n = 0; n < 4; ++n) {
         if (!((C.use >> n) & 1u)) continue;
         double lo, hi;
@@ -1031,6 +1030,12 @@ WQ_HD_INLINE bool block_cell_ok(const DetectCell& C, const TargetBox& B, bool br
         if (hi < -2.0 * C.W.m) return false;
     }
     return true;
+}
+
+WQ_HD_INLINE bool block_cell_ok(const DetectCell& C, const TargetBox& B, bool branch) {
+    if (!block_window_ok(C.W, B)) return false;
+    if (branch) return true;
+    return block_cell_nb_ok(C, B);
 }
 
 // The rays detect_eigenrays visits: interior and not on an edge.
@@ -1107,6 +1112,39 @@ WQ_HD_INLINE void store_window(const FanConst& F, const Arrays& A, const Slots& 
             for (int j = 0; j < 7; ++j) o[j * kWinField] = 0.0;
         }
     }
+}
+
+// The record of ray i in two parts for k_detect: the window form and width
+// (false for rays not searched), which the super-block's window bound
+// needs, and the rest - the previous-time form, the neighbour forms and the
+// neighbour bits - read only by wavefronts some of whose rays pass it.
+WQ_HD_INLINE bool load_cell_window(const FanConst& F, const Arrays& A, int64_t i, DetectCell& C) {
+    if (i < 0 || i >= F.N) return false;
+    const double* in = A.win + win_slot(F, i);
+    const double searched = in[15 * kWinField];
+    C.W.form.k = in[0];
+    for (int j = 0; j < 6; ++j) C.W.form.a[j] = in[(1 + j) * kWinField];
+    C.W.w = in[7 * kWinField];
+    C.W.m = F.lin_margin;
+    return searched != 0.0;
+}
+
+WQ_HD_INLINE void load_cell_rest(const FanConst& F, const Arrays& A, const Slots& S, int64_t i, DetectCell& C) {
+    if (i < 0 || i >= F.N) return;
+    const double* in = A.win + win_slot(F, i);
+    const double searched = in[15 * kWinField];
+    C.W.prev.k = in[8 * kWinField];
+    for (int j = 0; j < 6; ++j) C.W.prev.a[j] = in[(9 + j) * kWinField];
+    for (int n = 0; n < 4; ++n) {
+        C.nb[n].k = in[(16 + 7 * n) * kWinField];
+        for (int j = 0; j < 6; ++j) C.nb[n].a[j] = in[(17 + 7 * n + j) * kWinField];
+    }
+    const bool edge_lo = i - F.naz >= 0 && A.edge[at(S.curr, F.N, i - F.naz)] != 0;
+    const bool edge_hi = i + F.naz < F.N && A.edge[at(S.curr, F.N, i + F.naz)] != 0;
+    uint32_t use = searched != 0.0 ? (uint32_t)searched - 1u : 0u;
+    if ((use & 1u) && edge_lo) use &= ~1u;
+    if ((use & 2u) && edge_hi) use &= ~2u;
+    C.use = use;
 }
 
 // the whole record of ray i as a DetectCell; false for rays not searched.
