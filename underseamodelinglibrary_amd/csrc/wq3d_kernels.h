@@ -1089,12 +1089,15 @@ WQ_HD_INLINE void store_window(const FanConst& F, const Arrays& A, const Slots& 
     double* out = A.win + win_slot(F, i);
     DetectWindow W;
     if (F.nt <= 0 || !detect_ray_window(F, A, S, i, W)) {
+        out[7 * kWinField] = -HUGE_VAL;  // the width field marks the ray as not searched too
         out[15 * kWinField] = 0.0;
         return;
     }
     out[0] = W.form.k;
     for (int j = 0; j < 6; ++j) out[(1 + j) * kWinField] = W.form.a[j];
-    out[7 * kWinField] = W.w;
+    // a searched ray's width is never -inf (no window test tells -inf from
+    // the most negative finite width)
+    out[7 * kWinField] = W.w == -HUGE_VAL ? -0x1.fffffffffffffp+1023 : W.w;
     out[8 * kWinField] = W.prev.k;
     for (int j = 0; j < 6; ++j) out[(9 + j) * kWinField] = W.prev.a[j];
     int32_t nbj[4];
@@ -1115,18 +1118,18 @@ WQ_HD_INLINE void store_window(const FanConst& F, const Arrays& A, const Slots& 
 }
 
 // The record of ray i in two parts for k_detect: the window form and width
-// (false for rays not searched), which the super-block's window bound
-// needs, and the rest - the previous-time form, the neighbour forms and the
-// neighbour bits - read only by wavefronts some of whose rays pass it.
+// (false for rays not searched, whose width field is -inf), which the
+// super-block's window bound needs, and the rest - the previous-time form,
+// the neighbour forms and the neighbour bits - read only by wavefronts some
+// of whose rays pass it.
 WQ_HD_INLINE bool load_cell_window(const FanConst& F, const Arrays& A, int64_t i, DetectCell& C) {
     if (i < 0 || i >= F.N) return false;
     const double* in = A.win + win_slot(F, i);
-    const double searched = in[15 * kWinField];
     C.W.form.k = in[0];
     for (int j = 0; j < 6; ++j) C.W.form.a[j] = in[(1 + j) * kWinField];
     C.W.w = in[7 * kWinField];
     C.W.m = F.lin_margin;
-    return searched != 0.0;
+    return C.W.w != -HUGE_VAL;
 }
 
 WQ_HD_INLINE void load_cell_rest(const FanConst& F, const Arrays& A, const Slots& S, int64_t i, DetectCell& C) {
