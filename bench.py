@@ -30,7 +30,7 @@ W_BOX = 59.0              # FP64 ops per (ray, box) bound: the window form bound
 # HBM bytes per k_detect launch at this workload: rocprofv3 FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0 (factors from a 1 GiB device copy), means of 310 launches;
 # profiles/k_detect_single_fan_115k_rays_10k_targets.md holds the commands
-K_DETECT_TRAFFIC_BYTES = (21551.11 * 2.0 + 1353.02) * 1024.0
+K_DETECT_TRAFFIC_BYTES = (21861.09 * 2.0 + 1353.03) * 1024.0
 KERNELS = ["k_init", "(id 1, not launched)", "k_advance", "k_detect", "k_build", "(id 5, not launched)", "k_bounce"]
 
 
