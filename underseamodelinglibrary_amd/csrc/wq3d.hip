@@ -113,6 +113,37 @@ __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A
 }
 
 
+// The super-block bounds of every search tile, on the window fields of its
+// records read once: bit u of tile's mask is set when some ray of the tile
+// passes super-block u's window bound, so k_detect reads no record of a
+// tile for a super-block whose bit is clear.  One wavefront per tile; it
+// also counts the (ray, super-block) bounds tested.
+__global__ void __launch_bounds__(kBlock) k_supmask(FanConst F, Arrays A, Records Rc, int64_t tiles) {
+    const int lane = threadIdx.x & 63;
+    const int64_t tile = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (tile >= tiles) return;  // wave-uniform
+    const int64_t tiles_az = (F.naz + kTile - 1) / kTile;
+    const int64_t de = (tile / tiles_az) * kTile + lane / kTile;
+    const int64_t az = (tile % tiles_az) * kTile + lane % kTile;
+    const int64_t ray = (de < F.nde && az < F.naz) ? de * F.naz + az : -1;
+    DetectCell C;
+    const bool searched = load_cell_window(F, A, ray, C);
+    const unsigned long long n_searched = (unsigned long long)__popcll(__ballot(searched));
+    const TargetBox* boxes = reinterpret_cast<const TargetBox*>(F.tgt_box) + F.nblk;
+    const int64_t words = sup_words(F);
+    for (int64_t w0 = 0; w0 < words; ++w0) {
+        uint32_t bits = 0;
+        for (int b = 0; b < 32; ++b) {
+            const int64_t u = w0 * 32 + b;
+            if (u >= F.nsup) break;
+            if (__any(searched && block_window_ok(C.W, boxes[u]))) bits |= 1u << b;
+        }
+        if (lane == 0) A.tsup[tile * words + w0] = bits;
+    }
+    if (lane == 0 && n_searched)
+        atomicAdd(&Rc.scan[(blockIdx.x % kScanSlots) * 2 + 1], n_searched * (unsigned long long)F.nsup);
+}
+
 // one target's six features (tgt_lin rows)
 struct alignas(16) LinFeatures {
     double f[6];
@@ -146,11 +177,24 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
     const int64_t quad = L % kXcds + kXcds * (k / F.nsup);
     const int64_t sup = k % F.nsup;
     if (quad >= quads) return;  // whole workgroup, before the one barrier
+    // the four tiles' bits for this super-block: a workgroup none of whose
+    // tiles reaches it leaves before the barrier, a wavefront whose tile does
+    // not reads no record
+    const int64_t tiles_az = (F.naz + kTile - 1) / kTile;
+    const int64_t n_tiles = ((F.nde + kTile - 1) / kTile) * tiles_az;
+    const int64_t words = sup_words(F);
+    bool tile_in[kBlock / 64];
+    bool any_tile = false;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const int64_t tt = quad * (kBlock / 64) + w;
+        tile_in[w] = tt < n_tiles && ((A.tsup[tt * words + sup / 32] >> (sup % 32)) & 1u) != 0;
+        any_tile = any_tile || tile_in[w];
+    }
+    if (!any_tile) return;  // whole workgroup, before the one barrier
     // this lane's ray: tile of 8 AZ x 8 DE rays
     int64_t ray = -1;
     {
         const int64_t tile = quad * (kBlock / 64) + wv;
-        const int64_t tiles_az = (F.naz + kTile - 1) / kTile;
         const int64_t de = (tile / tiles_az) * kTile + lane / kTile;
         const int64_t az = (tile % tiles_az) * kTile + lane % kTile;
         if (de < F.nde && az < F.naz) ray = de * F.naz + az;
@@ -183,10 +227,10 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
     // the super-block's bound: the window part on the window fields of the
     // records, then the rest of the records for wavefronts still in
     DetectCell C;
-    bool active = load_cell_window(F, A, ray, C);
-    // (ray, box) bounds tested: the super-block's for every searched ray,
-    // each block's for the rays the super-block's bound kept
-    unsigned long long boxes_tested = (unsigned long long)__popcll(__ballot(active));
+    bool active = tile_in[wv] && load_cell_window(F, A, ray, C);
+    // (ray, box) bounds tested: each block's for the rays the super-block's
+    // bound kept (k_supmask counts the super-block bounds)
+    unsigned long long boxes_tested = 0;
     {
         const TargetBox SB = box_at(kSuperBlock);
         active = active && block_window_ok(C.W, SB);
@@ -463,6 +507,9 @@ void launch_advance(hipStream_t st, unsigned grid, const DEnv& E, const FanConst
 void launch_detect(hipStream_t st, int64_t quads, const FanConst& F, const Arrays& A, const Slots& S,
                    const Records& R) {
 #ifndef WQ3D_EMULATE
+    const int64_t tiles = win_tiles(F);
+    hipLaunchKernelGGL(k_supmask, dim3((unsigned)((tiles + kBlock / 64 - 1) / (kBlock / 64))), dim3(kBlock), 0, st, F,
+                       A, R, tiles);
     const int64_t grid = kXcds * ((quads + kXcds - 1) / kXcds) * F.nsup;
     hipLaunchKernelGGL(k_detect, dim3((unsigned)grid), dim3(kBlock), 0, st, F, A, S, R, quads);
 #else
@@ -1242,7 +1289,8 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         for (int32_t** p : ifields)
             if (f->mem.alloc(p, n4) != hipSuccess) return bail(fail(WQ3D_ERR_DEVICE, "state allocation failed"));
         if (f->mem.alloc(&A.edge, n4) != hipSuccess || f->mem.alloc(&A.flag, (size_t)N) != hipSuccess ||
-            f->mem.alloc(&A.win, (size_t)(win_tiles(F) * kWinField * kWinStride)) != hipSuccess)
+            f->mem.alloc(&A.win, (size_t)(win_tiles(F) * kWinField * kWinStride)) != hipSuccess ||
+            f->mem.alloc(&A.tsup, (size_t)(win_tiles(F) * std::max<int64_t>(1, sup_words(F)))) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "state allocation failed"));
     }
     // record buffers

@@ -51,6 +51,7 @@ struct Arrays {
     uint8_t* edge;
     uint8_t* flag;              // [N] reflection kind for the coming step
     double* win;                // [N][16] eigenray pre-test window of the step (store_window)
+    uint32_t* tsup;             // [tiles][sup_words(F)] bit u: some ray of the tile passes super-block u's window bound
 };
 
 struct FanConst {
@@ -1075,6 +1076,9 @@ constexpr int kWinTile = 8;
 WQ_HD_INLINE int64_t win_tiles(const FanConst& F) {
     return (int64_t)((F.nde + kWinTile - 1) / kWinTile) * ((F.naz + kWinTile - 1) / kWinTile);
 }
+
+// words of one tile's super-block bit mask
+WQ_HD_INLINE int64_t sup_words(const FanConst& F) { return (F.nsup + 31) / 32; }
 
 // offset of field 0 of ray i's record; field q is at q * kWinTile^2 past it
 WQ_HD_INLINE int64_t win_slot(const FanConst& F, int64_t i) {
