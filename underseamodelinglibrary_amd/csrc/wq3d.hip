@@ -119,26 +119,30 @@ __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A
 // tile for a super-block whose bit is clear.  One wavefront per tile; it
 // also counts the (ray, super-block) bounds tested.
 __global__ void __launch_bounds__(kBlock) k_supmask(FanConst F, Arrays A, Records Rc, int64_t tiles) {
+    // super-block boxes staged in LDS 32 at a time (one mask word)
+    __shared__ TargetBox s_sb[32];
     const int lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    if (tile >= tiles) return;  // wave-uniform
+    const bool have_tile = tile < tiles;  // wave-uniform; every thread stays for the barriers
     const int64_t tiles_az = (F.naz + kTile - 1) / kTile;
     const int64_t de = (tile / tiles_az) * kTile + lane / kTile;
     const int64_t az = (tile % tiles_az) * kTile + lane % kTile;
-    const int64_t ray = (de < F.nde && az < F.naz) ? de * F.naz + az : -1;
+    const int64_t ray = (have_tile && de < F.nde && az < F.naz) ? de * F.naz + az : -1;
     DetectCell C;
     const bool searched = load_cell_window(F, A, ray, C);
     const unsigned long long n_searched = (unsigned long long)__popcll(__ballot(searched));
-    const TargetBox* boxes = reinterpret_cast<const TargetBox*>(F.tgt_box) + F.nblk;
+    const double* boxes = F.tgt_box + F.nblk * 8;
     const int64_t words = sup_words(F);
     for (int64_t w0 = 0; w0 < words; ++w0) {
+        const int nb = (int)((F.nsup - w0 * 32 < 32) ? F.nsup - w0 * 32 : 32);
+        __syncthreads();
+        if (threadIdx.x < nb * 8)
+            reinterpret_cast<double*>(s_sb)[threadIdx.x] = boxes[w0 * 32 * 8 + threadIdx.x];
+        __syncthreads();
         uint32_t bits = 0;
-        for (int b = 0; b < 32; ++b) {
-            const int64_t u = w0 * 32 + b;
-            if (u >= F.nsup) break;
-            if (__any(searched && block_window_ok(C.W, boxes[u]))) bits |= 1u << b;
-        }
-        if (lane == 0) A.tsup[tile * words + w0] = bits;
+        for (int b = 0; b < nb; ++b)
+            if (__any(searched && block_window_ok(C.W, s_sb[b]))) bits |= 1u << b;
+        if (have_tile && lane == 0) A.tsup[tile * words + w0] = bits;
     }
     if (lane == 0 && n_searched)
         atomicAdd(&Rc.scan[(blockIdx.x % kScanSlots) * 2 + 1], n_searched * (unsigned long long)F.nsup);
