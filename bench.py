@@ -27,10 +27,11 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 W_TGT = 13.0              # FP64 ops per (target, ray) pair of the window test: 6 FMA + 1 compare
 W_BOX = 59.0              # FP64 ops per (ray, box) bound: the window form bounded over the box, 2 compares
-# HBM bytes per k_detect launch at this workload: rocprofv3 FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0 (factors from a 1 GiB device copy), means of 310 launches;
+# HBM bytes per step of the search stage (k_supmask + k_detect) at this
+# workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
+# 1 GiB device copy), means of 310 launches of each kernel;
 # profiles/k_detect_single_fan_115k_rays_10k_targets.md holds the commands
-K_DETECT_TRAFFIC_BYTES = (21861.09 * 2.0 + 1353.03) * 1024.0
+K_DETECT_TRAFFIC_BYTES = ((3733.50 + 21876.55) * 2.0 + (72.06 + 431.15)) * 1024.0
 KERNELS = ["k_init", "(id 1, not launched)", "k_advance", "k_detect", "k_build", "(id 5, not launched)", "k_bounce"]
 
 
@@ -181,7 +182,7 @@ def main():
         "eigenrays": int(len(rays)),
         "eigenrays_per_s": len(rays) / elapsed,
         "kernel_ms_per_launch": kernel_ms,
-        "roofline": {"bound": "mfma", "kernel": "k_detect", "achieved": achieved,
+        "roofline": {"bound": "mfma", "kernel": "k_supmask + k_detect (the search stage)", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": K_DETECT_TRAFFIC_BYTES,
                      "pairs_tested_per_launch": pairs_per_launch,
@@ -189,8 +190,9 @@ def main():
                      "pairs_all_per_launch": float(sc.num_targets() * n_rays),
                      "note": "FP64 vector (no MFMA-shaped work); achieved = (13 FP64 ops per "
                              "(target, ray) pair put to the window test + 59 per (ray, box) bound "
-                             "tested), both counted on the device, / mean launch time (HIP "
-                             "events); traffic in bytes per launch"},
+                             "tested), both counted on the device, / mean time per step of the "
+                             "search stage's two launches (HIP events around both); traffic in "
+                             "bytes per step of the stage"},
     }
     q.close()
     # the path's next caller stage: eigenray_collection's per-target totals

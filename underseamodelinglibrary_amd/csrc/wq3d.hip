@@ -1,6 +1,6 @@
 // wq3d.hip - HIP kernels (gfx950, FP64) and the C ABI of libwq3d.so.
 //
-// One wave_queue::step() (waveq3d/wave_queue.cc:155-192) is four launches on
+// One wave_queue::step() (waveq3d/wave_queue.cc:155-192) is five launches on
 // the fan's stream:
 //   k_bounce   detect_reflections without find_edges (wave_queue.cc:197-292):
 //              one thread per ray for the caustic and vertex counters, then
@@ -14,15 +14,17 @@
 //              gather), queue rotation, AB3 + update() of the new next slot
 //              (ode_integ.cc:131-186, wave_front.cc:125-178), attenuation and
 //              counter carry (wave_queue.cc:174-183), next step's flag[].
-//   k_detect   the eigenray search (wave_queue.cc:372-398, 411-478): for
-//              every (target, interior ray) pair a filter of linear forms in
-//              the target's features - the time-CPA pre-test as a window and
-//              the current-time comparison with the four in-plane
-//              neighbours - first per block of targets, then per pair; the
-//              few pairs it keeps get the exact pre-test and neighbour
-//              comparison, and the survivors go to the candidate list.
-//   k_build    is_closest_ray + build_eigenray (wave_queue.cc:411-702) with
-//              the hybrid Gaussian spreading walk per candidate.
+//   k_supmask  per 8 x 8 tile of rays, which super-blocks of targets some
+//              ray's pre-test window can reach.
+//   k_detect   the eigenray search filter (wave_queue.cc:372-398, 411-478):
+//              for every (target, interior ray) pair a filter of linear forms
+//              in the target's features - the time-CPA pre-test as a window
+//              and the current-time comparison with the four in-plane
+//              neighbours - first per box of targets, then per pair; the
+//              pairs it keeps go to the candidate list.
+//   k_build    the exact pre-test, is_closest_ray and build_eigenray
+//              (wave_queue.cc:384-702) with the hybrid Gaussian spreading
+//              walks per candidate.
 // Data layout: structure of arrays, slot-major ([slot][ray], ray = de*naz+az
 // with AZ fastest), frequency-major within a slot for attenuation/phase.
 // Queue rotation permutes slot indices on the host; nothing is copied.
