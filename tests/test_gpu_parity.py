@@ -268,3 +268,39 @@ def test_single_ray_reflection_known_answer(oracle_mod):
     assert len(oc_cb.events) == len(cb.events)
     for (tg, lg, yg), (to, lo, yo) in zip(cb.events, oc_cb.events):
         assert yg == yo and abs(tg - to) <= 1e-9 * abs(to) and abs(lg - lo) <= 1e-9
+
+
+def test_full_size_repeatable_and_sane():
+    """The benchmark's full fan (641 x 180 rays, 8 frequencies, 100 x 100
+    targets) for 150 steps, twice: the two runs' eigenray records are
+    identical bit for bit (the search's atomics and work order leave no
+    trace once the records are ordered), and every record is sane - target
+    indices in the grid, positive finite travel times, finite intensities
+    and phases, counters non-negative, launch DE inside the fan.  (Travel
+    times are not bounded by the run: next to the fan's open AZ seam,
+    AZ 358 -> 0, compute_offsets extrapolates far for targets in the gap,
+    as the oracle's restatement of the reference does too.)"""
+    sc = scenarios.config2_analytic()
+    steps = 150
+    print(f"[parity] full size: {sc.num_rays()} rays, {sc.num_targets()} targets, {steps} steps",
+          flush=True)
+    runs = []
+    for _ in range(2):
+        g = wave_queue(*sc.args())
+        g._lib.check(g._lib.wq3d_step(g._fan, steps))
+        runs.append(g.drain_eigenrays())
+        g.close()
+    (r0, i0, p0), (r1, i1, p1) = runs
+    assert len(r0) == len(r1) > 1000
+    for name in r0.dtype.names:
+        if name != "pad_":
+            np.testing.assert_array_equal(r0[name], r1[name], err_msg=name)
+    np.testing.assert_array_equal(i0, i1)
+    np.testing.assert_array_equal(p0, p1)
+    n1, n2 = sc.targets.size1(), sc.targets.size2()
+    assert (r0["t1"] >= 0).all() and (r0["t1"] < n1).all() and (r0["t2"] >= 0).all() and (r0["t2"] < n2).all()
+    assert (r0["travel_time"] > 0.0).all() and np.isfinite(r0["travel_time"]).all()
+    assert np.isfinite(i0).all() and np.isfinite(p0).all()
+    for name in ("surface", "bottom", "caustic", "upper", "lower"):
+        assert (r0[name] >= 0).all(), name
+    assert (r0["source_de"] >= -80.5).all() and (r0["source_de"] <= 80.5).all()
