@@ -306,8 +306,10 @@ wq3d_status wq3d_snapshot(wq3d_fan* fan, int32_t slot, wq3d_front_view* view);
 
 /* Average duration (ms) per launch of each kernel of the last step() calls,
  * measured with HIP events on the fan's stream when timing is enabled.
- * Kernel ids: 0 init, 1 (not launched), 2 advance, 3 detect, 4 build,
- * 5 (not launched), 6 bounce (caustics, vertices and reflections).
+ * Kernel ids: 0 init, 1 (not launched), 2 advance, 3 the eigenray search
+ * filter (its two launches, the tiles' super-block masks and the filter
+ * itself, timed together), 4 build, 5 (not launched), 6 bounce (caustics,
+ * vertices and reflections).
  * enable: 0 off, 1 every kernel, 2 the eigenray search (detect) only, which
  * keeps the events' own cost out of the other launches. */
 wq3d_status wq3d_set_timing(wq3d_fan* fan, int32_t enable);
