@@ -149,7 +149,11 @@ WQM_HD double reduce_pio2(double x, int* q) {
     double r = ::fma(-k, PIO2_1, x);
     r = ::fma(-k, PIO2_2, r);
     r = ::fma(-k, PIO2_3, r);
-    *q = (int)((long long)k & 3);
+    // k mod 4 in double (exact below 2^53), so the integer conversion is of
+    // a value in [0, 4) for every finite k; arguments this large are far
+    // outside the reduction's accurate range anyway
+    const double k4 = k - 4.0 * ::floor(k * 0.25);
+    *q = (k4 >= 0.0 && k4 < 4.0) ? (int)k4 : 0;
     return r;
 }
 
@@ -221,6 +225,10 @@ WQM_HD double atan_unit(double t) {
         abort();
     }
 #endif
+    // the table index in double first: t is in [0, 1] for every caller, and
+    // a NaN t (atan2 of two infinities) yields NaN, never an index outside
+    // the table
+    if (!(t >= 0.0 && t <= 1.0)) return t - t;
     const int k = (int)::floor(t * 4.0 + 0.5);
     const double c = 0.25 * (double)k;
     const double u = (t - c) / ::fma(t, c, 1.0);

@@ -87,18 +87,22 @@ struct Seq {
     size_t size() const { return v.size(); }
     double operator()(size_t i) const { return v[std::min(max_index, i)]; }
     double increment(size_t i) const { return inc[std::min(max_index, i)]; }
+    // The reference's (difference_type)floor(q) clamped to [0, size-2]; a
+    // NaN q or one at or above 2^63 converts to INT64_MIN on x86-64 (the
+    // "integer indefinite" value) and so clamps to 0.  The clamp is made in
+    // double here so the conversion is defined for every input.
+    static size_t clamp_floor(double q, long n2) {
+        if (!(q < 0x1p63) || q <= 0.0) return 0;
+        if (q >= (double)n2) return (size_t)std::max(0L, n2);
+        return (size_t)q;
+    }
     size_t find_index(double value) const {
         const long n2 = (long)size() - 2;
-        if (search == WQ3D_SEQ_LINEAR) {
-            long k = (long)std::floor((value - v[0]) / inc[0]);
-            return (size_t)std::max(0L, std::min(n2, k));
-        }
-        if (search == WQ3D_SEQ_LOG) {
-            long k = (long)std::floor(wqm::log(value / v[0]) / wqm::log(inc[1] / inc[0]));
-            return (size_t)std::max(0L, std::min(n2, k));
-        }
+        if (search == WQ3D_SEQ_LINEAR) return clamp_floor(std::floor((value - v[0]) / inc[0]), n2);
+        if (search == WQ3D_SEQ_LOG)
+            return clamp_floor(std::floor(wqm::log(value / v[0]) / wqm::log(inc[1] / inc[0])), n2);
         value *= sign;
-        if (max_index < 1) return 0;
+        if (max_index < 1 || !(value == value)) return 0;
         if (value <= sign * v[0]) return 0;
         if (value >= sign * v[max_index - 1]) return max_index - 1;
         size_t n = 0;  // first entry with key > value (std::map::upper_bound)

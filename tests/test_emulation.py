@@ -6,6 +6,9 @@ with the oracle, including the gridded sound-speed / bathymetry path that is
 switched off on the GPU.  Each case runs in a child process with the ASan
 runtime preloaded.
 """
+import os
+import subprocess
+import sys
 import textwrap
 
 import pytest
@@ -75,6 +78,102 @@ def test_emulated_config2_gridded_exact():
         print("eigenrays", run(sc))
     """)
     assert int(out.split("eigenrays")[-1]) > 500
+
+
+GRIDDED_CASE = textwrap.dedent("""
+    sc = scenarios.config2(de=seq_linear(-80.0, 2.0, 80.0), az=seq_linear(0.0, 10.0, 350.0),
+                           n_targets=(10, 10), time_max=STEPS_S, freq=seq_log(100.0, 10.0, 3))
+    print("eigenrays", run(sc, verbs=True))
+""")
+
+
+@pytest.mark.parametrize("order", ["reverse", "phased"])
+def test_emulated_config2_gridded_launch_orders(order):
+    """The gridded case with the rays of each launch visited last to first,
+    and with every ray's caustic / vertex part before any ray's reflections:
+    a kernel whose result depended on the order of its threads (a neighbour
+    read racing a neighbour write) would differ.  Bit-identical to the
+    oracle, eigenverbs and reflections included; every pointer each launch
+    hands its kernel is checked against the emulated device allocations."""
+    out = _run(f"import os\nos.environ['WQ3D_EMU_ORDER'] = '{order}'\n" +
+               GRIDDED_CASE.replace("STEPS_S", "30.0"))
+    assert int(out.split("eigenrays")[-1]) > 500
+
+
+def test_emulated_config2_gridded_split_lanes():
+    """The GPU form of eigenray construction (a group of 32 lanes per
+    candidate, cross-lane shuffles, an LDS cube, walk cells fetched ahead by
+    the lanes) with the lanes as host threads in lockstep: every group
+    operation is a barrier that all lanes must reach from the same call site
+    of the same candidate.  Bit-identical to the oracle on the gridded case."""
+    out = _run("import os\nos.environ['WQ3D_EMU_SPLIT'] = '1'\n" + GRIDDED_CASE.replace("STEPS_S", "15.0"))
+    assert int(out.split("eigenrays")[-1]) > 100
+
+
+def test_emulated_config2_gridded_ubsan(tmp_path):
+    """The gridded case under UndefinedBehaviorSanitizer with
+    float-cast-overflow: no float-to-integer conversion of the kernels is
+    out of range (the indices of grid lookups and table reads come from
+    such conversions), both construction forms."""
+    from underseamodelinglibrary_amd import build
+    lib = str(tmp_path / "libwq3d_ubsan.so")
+    subprocess.run([build.HIPCC, "--offload-host-only", "-O1", "-g", "-std=c++17", "-fPIC", "-shared",
+                    "-ffp-contract=off", "-fno-fast-math", "-DWQ3D_EMULATE", "-Xarch_host",
+                    "-fsanitize=undefined,float-cast-overflow", "-Xarch_host", "-fno-sanitize-recover=all",
+                    "-shared-libsan", "-I", os.path.join(emulation.ROOT, "include"), "-o", lib, build.SRC],
+                   check=True, capture_output=True)
+    rt = subprocess.run(["/opt/rocm/bin/amdclang++", "-print-file-name=libclang_rt.ubsan_standalone-x86_64.so"],
+                        capture_output=True, text=True, check=True).stdout.strip()
+    script = COMMON.replace("lib = emulation.load()", f"import ctypes as C\nfrom underseamodelinglibrary_amd "
+                            f"import _lib as _L\nlib = _L.bind(C.CDLL({lib!r}))")
+    for split in ("0", "1"):
+        env = dict(os.environ, LD_PRELOAD=rt, UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1",
+                   WQ3D_EMU_SPLIT=split,
+                   PYTHONPATH=os.pathsep.join([emulation.ROOT, os.path.join(emulation.ROOT, "oracle"),
+                                               os.path.join(emulation.ROOT, "tests")]))
+        res = subprocess.run([sys.executable, "-c", script + GRIDDED_CASE.replace("STEPS_S", "10.0")],
+                             env=env, capture_output=True, text=True, timeout=900)
+        assert res.returncode == 0 and "runtime error" not in res.stderr, res.stdout[-2000:] + res.stderr[-4000:]
+        assert int(res.stdout.split("eigenrays")[-1]) > 10
+
+
+def test_find_index_defined_for_every_input():
+    """find_index of the kernels (seq_linear / seq_log / seq_data searches)
+    for NaN, infinities and values far outside the axis: always an index in
+    [0, size-2], equal to the reference's max(0, min(size-2,
+    (int64)floor(q))) as x86-64 evaluates it (numpy's float64 -> int64 cast
+    uses the same conversion, which gives INT64_MIN for NaN and |q| >= 2^63)."""
+    out = _run("""
+        import ctypes as C, warnings
+        import numpy as np
+        fn = lib.wq3d_emu_find_index
+        fn.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_double]
+        fn.restype = C.c_int32
+        probes = [float("nan"), float("inf"), -float("inf"), 1e300, -1e300, 1e18, -1e18, 1e15, -1e15,
+                  0.0, 1.0, -1.0]
+        bad = 0
+        checked = 0
+        for axis, search in ((seq_linear(6371000.0, -128.2, 40), 0), (seq_linear(-0.7, 0.004, 41), 0),
+                             (seq_log(100.0, 10.0 ** (2.0 / 7.0), 8), 1)):
+            v, inc = axis.values, axis.increments
+            n = v.size
+            xs = probes + list(v) + [float(x) for x in (v[:-1] + v[1:]) / 2] + [v[0] * 1.0001, v[-1] * 1.0001]
+            for x in xs:
+                k = fn(v.ctypes.data_as(C.POINTER(C.c_double)), inc.ctypes.data_as(C.POINTER(C.c_double)),
+                       n, search, x)
+                with np.errstate(all="ignore"), warnings.catch_warnings():
+                    warnings.simplefilter("ignore")
+                    q = (np.floor((np.float64(x) - v[0]) / inc[0]) if search == 0 else
+                         np.floor(np.log(np.float64(x) / v[0]) / np.log(inc[1] / inc[0])))
+                    ref = max(0, min(n - 2, int(np.array([q]).astype(np.int64)[0])))
+                checked += 1
+                if not (0 <= k <= n - 2) or (search == 0 and k != ref):
+                    bad += 1
+                    print("mismatch", x, k, ref)
+        print("counts", checked, bad)
+    """)
+    checked, bad = (int(x) for x in out.split("counts")[-1].split())
+    assert checked > 100 and bad == 0
 
 
 def test_emulated_eigenverbs_volume_exact():

@@ -107,9 +107,6 @@ def test_multifrequency_fan_parity(oracle_mod):
     assert _check_all(g, o, sc.freq.size()) > 1000
 
 
-@pytest.mark.skip(reason="gridded environments are disabled on the GPU: their first parity run "
-                         "faulted the device and no cause has been shown; the same kernel bodies "
-                         "are checked against the oracle in tests/test_emulation.py")
 def test_config2_reduced_parity(oracle_mod):
     """Gridded sound speed (pchip x linear x linear) and bathymetry with a
     reduced fan and target grid."""

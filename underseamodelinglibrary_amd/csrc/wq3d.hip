@@ -37,6 +37,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -382,6 +383,7 @@ inline hipError_t last_error() { return hipGetLastError(); }
 inline hipError_t malloc(void** p, size_t n) {
     *p = std::malloc(n);
     if (!*p) return hipErrorOutOfMemory;
+    emu::note_alloc(*p, n);
     const char* fill = std::getenv("WQ3D_EMU_FILL");
     if (fill && std::strcmp(fill, "none") == 0) return hipSuccess;  // left uninitialised
     if (fill && std::strcmp(fill, "random") == 0) {
@@ -397,6 +399,7 @@ inline hipError_t malloc(void** p, size_t n) {
     return hipSuccess;
 }
 inline hipError_t free(void* p) {
+    emu::note_free(p);
     std::free(p);
     return hipSuccess;
 }
@@ -439,6 +442,110 @@ inline hipError_t last_error() { return hipSuccess; }
 #endif
 }  // namespace rt
 
+#ifdef WQ3D_EMULATE
+// Emulation build: every pointer a kernel dereferences must lie inside a
+// "device" allocation, with the extent the kernel assumes (wq3d_emulate.h).
+template <class T>
+void emu_range(const T* p, int64_t count, const char* what) {
+    if (count > 0) emu::check_range(p, (size_t)count * sizeof(T), what);
+}
+
+void emu_check_seq(const DSeq& q, const char* what) {
+    emu_range(q.v, q.n, what);
+    emu_range(q.inc, q.n, what);
+}
+
+void emu_check_grid(const DGrid& g, const char* what) {
+    int64_t total = 1;
+    for (int d = 0; d < g.nd; ++d) {
+        emu_check_seq(g.ax[d], what);
+        total *= g.ax[d].n;
+    }
+    if (total != g.size) {
+        std::fprintf(stderr, "wq3d emulation check failed: %s grid size %lld, axes give %lld\n", what,
+                     (long long)g.size, (long long)total);
+        std::abort();
+    }
+    emu_range(g.data, g.size, what);
+}
+
+void emu_check_env(const DEnv& E) {
+    if (E.profile.kind == WQ3D_PROFILE_GRID) emu_check_grid(E.profile.grid, "sound speed grid");
+    if (E.surface.kind == 1) emu_check_grid(E.surface.grid, "surface grid");
+    if (E.bottom.kind == 1) emu_check_grid(E.bottom.grid, "bathymetry grid");
+}
+
+void emu_check_fan(const FanConst& F) {
+    emu_check_seq(F.freq, "F.freq");
+    emu_check_seq(F.de, "F.de");
+    emu_check_seq(F.az, "F.az");
+    emu_range(F.cos_de, F.nde, "F.cos_de");
+    emu_range(F.sin_de, F.nde, "F.sin_de");
+    emu_range(F.cos_az, F.naz, "F.cos_az");
+    emu_range(F.sin_az, F.naz, "F.sin_az");
+    const int64_t nt = F.nt;
+    const double* td[] = {F.tgt_r, F.tgt_t, F.tgt_p, F.tgt_s, F.tgt_th, F.tgt_ph, F.tgt_r2, F.tgt_2r};
+    for (const double* q : td) emu_range(q, nt, "F.tgt_*");
+    emu_range(F.tgt_branch, nt, "F.tgt_branch");
+    emu_range(F.tgt_lin, 6 * nt, "F.tgt_lin");
+    emu_range(F.tgt_perm, nt, "F.tgt_perm");
+    emu_range(F.scan_branch, nt, "F.scan_branch");
+    if (nt > 0) {
+        emu_range(F.tgt_box, 8 * (F.nblk + F.nsup), "F.tgt_box");
+        emu_range(F.blk_branch, F.nblk + F.nsup, "F.blk_branch");
+    }
+    emu_range(F.norm_de, F.nde, "F.norm_de");
+    emu_range(F.norm_az, (int64_t)F.nde * F.naz, "F.norm_az");
+}
+
+void emu_check_arrays(const FanConst& F, const Arrays& A) {
+    const int64_t n4 = 4 * F.N;
+    const double* d[] = {A.pr, A.pt, A.pp, A.gr, A.gt, A.gp, A.nr, A.nt, A.np, A.hr, A.ht,
+                         A.hp, A.c,  A.cr, A.ct, A.cp, A.dist, A.path, A.tr, A.tt, A.tp, A.ts};
+    for (const double* q : d) emu_range(q, n4, "A.<state field>");
+    emu_range(A.atten, n4 * F.nf, "A.atten");
+    emu_range(A.phase, n4 * F.nf, "A.phase");
+    const int32_t* c[] = {A.srf, A.btm, A.cst, A.upr, A.lwr};
+    for (const int32_t* q : c) emu_range(q, n4, "A.<counter>");
+    emu_range(A.edge, n4, "A.edge");
+    emu_range(A.flag, F.N, "A.flag");
+    emu_range(A.win, win_tiles(F) * kWinField * kWinStride, "A.win");
+    emu_range(A.tsup, win_tiles(F) * std::max<int64_t>(1, sup_words(F)), "A.tsup");
+}
+
+void emu_check_records(const FanConst& F, const Records& R) {
+    emu_range(R.rays, R.ray_cap, "R.rays");
+    emu_range(R.ray_int, R.ray_cap * F.nf, "R.ray_int");
+    emu_range(R.ray_phase, R.ray_cap * F.nf, "R.ray_phase");
+    emu_range(R.verbs, R.verb_cap, "R.verbs");
+    emu_range(R.verb_power, R.verb_cap * F.nf, "R.verb_power");
+    emu_range(R.refl, R.refl_cap, "R.refl");
+    unsigned long long* const cnt[] = {R.ray_count, R.verb_count, R.refl_count, R.hit_count, R.pass_count,
+                                       R.cand_count};
+    for (unsigned long long* q : cnt) emu_range(q, 1, "R.<counter>");
+    emu_range(R.cand, R.cand_cap, "R.cand");
+    emu_range(R.scan, 2 * kScanSlots, "R.scan");
+    emu_range(R.error, 1, "R.error");
+}
+
+void emu_check_launch(const DEnv& E, const FanConst& F, const Arrays& A, const Records* R) {
+    emu_check_env(E);
+    emu_check_fan(F);
+    emu_check_arrays(F, A);
+    if (R) emu_check_records(F, *R);
+}
+
+// the order in which the serial emulation visits the rays of a launch:
+// WQ3D_EMU_ORDER=reverse runs them last to first.  A kernel whose result
+// depends on the order of its threads would differ between the two.
+bool emu_reverse() {
+    const char* o = std::getenv("WQ3D_EMU_ORDER");
+    return o && std::strcmp(o, "reverse") == 0;
+}
+
+int64_t emu_ray(int64_t k, int64_t n) { return emu_reverse() ? n - 1 - k : k; }
+#endif
+
 // kernel launches (serial loops over the same bodies in the emulation build)
 #ifndef WQ3D_EMULATE
 // the compile-time environment variant for E (see KindsOf): 0 generic,
@@ -469,7 +576,8 @@ void launch_init(hipStream_t st, unsigned grid, const DEnv& E, const FanConst& F
 #ifndef WQ3D_EMULATE
     WQ3D_LAUNCH_ENV(k_init, E, grid, st, F, A, S);
 #else
-    for (int64_t i = 0; i < F.N; ++i) body_init(i, E, F, A, S);
+    emu_check_launch(E, F, A, nullptr);
+    for (int64_t k = 0; k < F.N; ++k) body_init(emu_ray(k, F.N), E, F, A, S);
 #endif
 }
 
@@ -487,13 +595,27 @@ void launch_bounce(hipStream_t st, unsigned grid, const KParams* kp, const DEnv&
         hipLaunchKernelGGL((k_bounce<-1, false>), dim3(grid), dim3(kBlock), 0, st, kp, S, t, key);
     }
 #else
-    (void)kp;
-    *R.cand_count = 0ull;
-    *R.hit_count = 0ull;
-    *R.pass_count = 0ull;
-    for (int64_t i = 0; i < F.N; ++i) {
-        body_reflect_light(i, F, A, S);
-        body_bounce(i, E, F, A, S, R, t, key);
+    // the GPU kernel reads its parameters from the fan's device copy; so
+    // does the emulation, after checking that copy and what it points to
+    (void)E; (void)F; (void)A; (void)R;
+    emu::check_range(kp, sizeof(KParams), "KParams");
+    const KParams* P = kp;
+    emu_check_launch(P->E, P->F, P->A, &P->R);
+    *P->R.cand_count = 0ull;
+    *P->R.hit_count = 0ull;
+    *P->R.pass_count = 0ull;
+    // WQ3D_EMU_ORDER=phased runs every ray's caustic and vertex part before
+    // any ray's reflections (on the GPU both orders happen between threads)
+    const char* order = std::getenv("WQ3D_EMU_ORDER");
+    if (order && std::strcmp(order, "phased") == 0) {
+        for (int64_t i = 0; i < P->F.N; ++i) body_reflect_light(i, P->F, P->A, S);
+        for (int64_t i = 0; i < P->F.N; ++i) body_bounce(i, P->E, P->F, P->A, S, P->R, t, key);
+    } else {
+        for (int64_t k = 0; k < P->F.N; ++k) {
+            const int64_t i = emu_ray(k, P->F.N);
+            body_reflect_light(i, P->F, P->A, S);
+            body_bounce(i, P->E, P->F, P->A, S, P->R, t, key);
+        }
     }
 #endif
 }
@@ -503,7 +625,9 @@ void launch_advance(hipStream_t st, unsigned grid, const DEnv& E, const FanConst
 #ifndef WQ3D_EMULATE
     WQ3D_LAUNCH_ENV(k_advance, E, grid, st, F, A, S);
 #else
-    for (int64_t i = 0; i < F.N; ++i) {
+    emu_check_launch(E, F, A, nullptr);
+    for (int64_t k = 0; k < F.N; ++k) {
+        const int64_t i = emu_ray(k, F.N);
         body_advance(i, E, F, A, S);
         store_window(F, A, S, i);
     }
@@ -519,6 +643,7 @@ void launch_detect(hipStream_t st, int64_t quads, const FanConst& F, const Array
     const int64_t grid = kXcds * ((quads + kXcds - 1) / kXcds) * F.nsup;
     hipLaunchKernelGGL(k_detect, dim3((unsigned)grid), dim3(kBlock), 0, st, F, A, S, R, quads);
 #else
+    emu_check_launch(DEnv{}, F, A, &R);
     const TargetBox* boxes = reinterpret_cast<const TargetBox*>(F.tgt_box);
     const bool trace = std::getenv("WQ3D_EMU_TRACE") != nullptr;
     int64_t rays = 0, rays_kept = 0, ray_blocks = 0, window_blocks = 0, cell_blocks = 0;
@@ -575,9 +700,39 @@ void launch_build(hipStream_t st, const KParams* kp, const DEnv& E, const FanCon
     while (group < F.nf) group <<= 1;
     hipLaunchKernelGGL(k_build, dim3(1024), dim3(kBlock), 0, st, kp, S, t, key, group);
 #else
-    (void)kp;
-    const int64_t n = (int64_t)std::min<unsigned long long>(*R.cand_count, (unsigned long long)R.cand_cap);
-    for (int64_t c = 0; c < n; ++c) body_build<false>(c, -1, 1, nullptr, E, F, A, S, R, t, key);
+    (void)E; (void)F; (void)A; (void)R;
+    emu::check_range(kp, sizeof(KParams), "KParams");
+    const KParams* P = kp;
+    emu_check_launch(P->E, P->F, P->A, &P->R);
+    const Records& Rc = P->R;
+    const int64_t n = (int64_t)std::min<unsigned long long>(*Rc.cand_count, (unsigned long long)Rc.cand_cap);
+    const char* split = std::getenv("WQ3D_EMU_SPLIT");
+    if (!(split && std::strcmp(split, "1") == 0)) {
+        for (int64_t c = 0; c < n; ++c) body_build<false>(c, -1, 1, nullptr, P->E, P->F, P->A, S, Rc, t, key);
+        return;
+    }
+    // the GPU form (body_build<true>): each candidate built by a group of
+    // `group` lanes, here host threads in lockstep, sharing one LDS area
+    int group = kBuildLanes;
+    while (group < P->F.nf) group <<= 1;
+    emu::Group grp;
+    grp.G = group;
+    std::vector<double> lds(54, 0.0);
+    auto lane_main = [&](int g) {
+        emu::LaneState& ls = emu::lane_state();
+        ls.group = &grp;
+        ls.lane = g;
+        for (int64_t c = 0; c < n; ++c) {
+            ls.tag = c;
+            body_build<true>(c, g, group, group >= 4 ? lds.data() : nullptr, P->E, P->F, P->A, S, Rc, t, key);
+            grp.barrier(g, -1, c);  // the next candidate starts together
+        }
+        ls.group = nullptr;
+    };
+    std::vector<std::thread> lanes;
+    for (int g = 1; g < group; ++g) lanes.emplace_back(lane_main, g);
+    lane_main(0);
+    for (std::thread& th : lanes) th.join();
 #endif
 }
 
@@ -797,6 +952,7 @@ wq3d_status grid_upload(DeviceBuffers& mem, const wq3d_grid& g, DGrid& dg, DGrid
     }
     store.emplace_back(g.data, g.data + total);
     hg.data = store.back().data();
+    dg.size = hg.size = (int64_t)total;
     if (mem.upload(&dg.data, g.data, total) != hipSuccess)
         return fail(WQ3D_ERR_DEVICE, "grid data upload failed");
     return WQ3D_OK;
@@ -984,16 +1140,6 @@ wq3d_status wq3d_env_create(const wq3d_env_desc* desc, wq3d_env** out) {
     if (!desc || !out) return fail(WQ3D_ERR_ARGUMENT, "null argument");
     *out = nullptr;
     if (wq3d_device_count() < 1) return fail(WQ3D_ERR_DEVICE, "no HIP device");
-#ifndef WQ3D_EMULATE
-    // Gridded sound speed and bathymetry are switched off on the GPU: their
-    // first parity run ended in a device memory fault whose cause has not
-    // been found (the same kernel bodies run clean and exact against the
-    // oracle in the bounds-checked CPU build).  They stay rejected until it is.
-    if (desc->profile.kind == WQ3D_PROFILE_GRID || desc->bottom.kind == WQ3D_BOUNDARY_GRID)
-        return fail(WQ3D_ERR_UNSUPPORTED,
-                    "gridded sound speed / bathymetry are disabled on the GPU "
-                    "(unresolved device fault; CPU-checked only)");
-#endif
     auto* env = new wq3d_env();
     HIP_TRY(rt::get_device(&env->device));
     const wq3d_profile_desc& p = desc->profile;
@@ -1713,6 +1859,23 @@ wq3d_status wq3d_kernel_times(wq3d_fan* f, double* ms, int64_t* launches, int32_
 }
 
 #ifdef WQ3D_EMULATE
+// Test hook of the CPU emulation build only: find_index of the kernels on a
+// host sequence, without the emulation's abort on non-finite arguments, so
+// a test can check the index for every input (NaN, infinities, values far
+// outside the axis).
+int32_t wq3d_emu_find_index(const double* value, const double* increment, int32_t n, int32_t search, double x) {
+    DSeq q;
+    q.v = value;
+    q.inc = increment;
+    q.n = n;
+    q.search = search;
+    q.sign = (n > 1 && value[1] < value[0]) ? -1.0 : 1.0;
+    wq3d_emulation_strict() = false;
+    const int k = find_index(q, x);
+    wq3d_emulation_strict() = true;
+    return k;
+}
+
 // Test hook of the CPU emulation build only: random rays and targets near
 // the earth's radius, plus targets placed by bisection on the boundaries
 // where an exact comparison of the pre-test flips, decided once with the

@@ -13,6 +13,7 @@
 
 #include "../../include/wq3d.h"
 #include "wq3d_device.h"
+#include "wq3d_emulate.h"
 
 #define WQ_HD __host__ __device__
 #define WQ_HD_INLINE __host__ __device__ __forceinline__
@@ -1472,6 +1473,18 @@ __device__ __forceinline__ unsigned long long group_bcast(unsigned long long v, 
     const unsigned int hi = (unsigned int)__shfl((int)(unsigned int)(v >> 32), base);
     return ((unsigned long long)hi << 32) | lo;
 }
+#elif defined(WQ3D_EMULATE)
+// emulation build: the lanes of a group are host threads in lockstep
+// (wq3d_emulate.h); a single thread (the serial build) is its own group
+inline bool group_any(bool p, int, int where = __builtin_LINE()) {
+    return emu::in_group() ? emu::group_any(p, where) : p;
+}
+inline unsigned long long group_bcast(unsigned long long v, int, int where = __builtin_LINE()) {
+    return emu::in_group() ? emu::group_bcast(v, where) : v;
+}
+inline double group_shfl(double v, int src, int, int where = __builtin_LINE()) {
+    return emu::in_group() ? emu::group_shfl(v, src, where) : v;
+}
 #else
 inline bool group_any(bool p, int) { return p; }
 inline unsigned long long group_bcast(unsigned long long v, int) { return v; }
@@ -1634,11 +1647,16 @@ WQ_HD_INLINE CellStream<Split, CellOf> cell_stream(const FreqSel<Split>& fs, Cel
 
 // Orders the LDS writes of a wavefront's lanes before their reads by other
 // lanes of the same wavefront.
-WQ_HD_INLINE void wave_sync() {
+WQ_HD_INLINE void wave_sync(int where = __builtin_LINE()) {
 #if defined(__HIP_DEVICE_COMPILE__)
+    (void)where;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#elif defined(WQ3D_EMULATE)
+    if (emu::in_group()) emu::wave_sync(where);
+#else
+    (void)where;
 #endif
 }
 
