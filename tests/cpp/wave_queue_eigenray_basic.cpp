@@ -22,8 +22,9 @@ struct printer : waveq3d::eigenray_listener {
 int main() {
     try {
         types::compute_earth_radius(45.0);
-        ocean::ocean_model ocean(ocean::boundary_flat(), ocean::boundary_flat(3000.0),
-                                 ocean::profile_linear(1500.0, ocean::attenuation_constant(0.0)));
+        auto ocean = std::make_shared<const ocean::ocean_model>(
+            ocean::boundary_flat(), ocean::boundary_flat(3000.0),
+            ocean::profile_linear(1500.0, ocean::attenuation_constant(0.0)));
         auto freq = std::make_shared<types::seq_log>(10e3, 2.0, (size_t)3);
         types::wposition1 pos(45.0, -45.0, -1000.0);
         auto de = std::make_shared<types::seq_linear>(-60.0, 5.0, 60.0);
@@ -37,6 +38,9 @@ int main() {
         wave.add_eigenray_listener(&collection);
         while (wave.time() < 3.5) wave.step();
         std::printf("checks %d time %.17g\n", p.checks, wave.time());
+        // clamped sequence accessors: a negative index is a huge size_t in the
+        // reference and clamps to the last element (types/seq_vector.h:98-101)
+        std::printf("seq %.17g %.17g %.17g %.17g\n", (*de)(-1), de->increment(-1), (*de)(1000), (*de)(0));
         collection.sum_eigenrays();
         const auto total = collection.total(0, 0);
         std::printf("total %zu %.17g %.17g %.17g %.17g %.17g %d %d %.17g %llu\n", collection.num_eigenrays(),

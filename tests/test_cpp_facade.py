@@ -15,7 +15,7 @@ from underseamodelinglibrary_amd import build, scenarios
 
 
 def _parse(out):
-    rays, checks, classic, total = [], None, None, None
+    rays, checks, classic, total, seq = [], None, None, None, None
     for line in out.splitlines():
         f = line.split()
         if f[0] == "total":
@@ -30,12 +30,17 @@ def _parse(out):
             checks = int(f[1])
         elif f[0] == "classic":
             classic = f[1]
-    return rays, checks, classic, total
+        elif f[0] == "seq":
+            seq = [float(x) for x in f[1:]]
+    return rays, checks, classic, total, seq
 
 
 def _check(out, oracle_mod):
-    rays, checks, classic, total = _parse(out)
+    rays, checks, classic, total, seq = _parse(out)
     assert classic == "rejected"
+    # DE fan -60:5:60: index -1 and 1000 both clamp to the last element (60),
+    # index 0 is the first (-60)
+    assert seq == [60.0, 5.0, 60.0, -60.0]
     sc = scenarios.eigenray_basic()
     assert checks == sc.steps
     o = oracle_mod.oracle_queue(*sc.args())

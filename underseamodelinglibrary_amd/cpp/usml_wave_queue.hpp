@@ -80,9 +80,10 @@ public:
     wq3d_seq abi() const { return wq3d_seq{(int64_t)_v.size(), _v.data(), _inc.data(), _search, 0}; }
 
 protected:
+    // the reference's index is a size_t: a negative index wraps to a huge
+    // value and clamps to the last element, like one past the end
     size_t clamp(long i) const {
-        if (i < 0) return 0;
-        return (size_t)i >= _v.size() ? _v.size() - 1 : (size_t)i;
+        return (i < 0 || (size_t)i >= _v.size()) ? _v.size() - 1 : (size_t)i;
     }
     std::vector<double> _v, _inc;
     int32_t _search = WQ3D_SEQ_DATA;
@@ -333,6 +334,7 @@ struct ocean_model {
     std::vector<volume_flat> volumes;
     ocean_model(boundary_model s, boundary_model b, profile_model p) : surface(s), bottom(b), profile(p) {}
     void add_volume(volume_flat v) { volumes.push_back(v); }
+    using csptr = std::shared_ptr<const ocean_model>;  // ocean/ocean_model.h:38
 };
 
 }  // namespace ocean
@@ -400,14 +402,15 @@ struct wave_front {
 
 class wave_queue {
 public:
-    wave_queue(const ocean::ocean_model& ocean, std::shared_ptr<const types::seq_vector> freq,
+    // waveq3d/wave_queue.h:122-126: the ocean is shared and immutable
+    wave_queue(const ocean::ocean_model::csptr& ocean, std::shared_ptr<const types::seq_vector> freq,
                const types::wposition1& pos, std::shared_ptr<const types::seq_vector> de,
                std::shared_ptr<const types::seq_vector> az, double time_step,
                const types::wposition* target_pos = nullptr, spreading_type type = HYBRID_GAUSSIAN)
-        : _freq(std::move(freq)), _de(std::move(de)), _az(std::move(az)), _time_step(time_step),
+        : _ocean(ocean), _freq(std::move(freq)), _de(std::move(de)), _az(std::move(az)), _time_step(time_step),
           _targets(target_pos) {
-        if (!_freq || !_de || !_az) throw std::invalid_argument("null sequence");
-        build_env(ocean);
+        if (!_ocean || !_freq || !_de || !_az) throw std::invalid_argument("null ocean or sequence");
+        build_env(*_ocean);
         wq3d_fan_desc d{};
         d.freq = _freq->abi();
         d.de = _de->abi();
@@ -668,6 +671,7 @@ private:
         return w;
     }
 
+    ocean::ocean_model::csptr _ocean;
     std::shared_ptr<const types::seq_vector> _freq, _de, _az;
     double _time_step;
     const types::wposition* _targets;

@@ -48,13 +48,17 @@ class seq_vector:
     def __len__(self):
         return self.size()
 
+    def _clamp(self, index):
+        # the reference's index is a size_t: a negative index wraps to a huge
+        # value and clamps to the last element (types/seq_vector.h:98-101)
+        index = int(index)
+        return self.size() - 1 if index < 0 or index > self.size() - 1 else index
+
     def __call__(self, index):
-        index = min(max(int(index), 0), self.size() - 1)
-        return float(self._v[index])
+        return float(self._v[self._clamp(index)])
 
     def increment(self, index):
-        index = min(max(int(index), 0), self.size() - 1)
-        return float(self._inc[index])
+        return float(self._inc[self._clamp(index)])
 
     @property
     def values(self):
