@@ -301,6 +301,21 @@ wq3d_status wq3d_sum_eigenrays(const wq3d_eigenray* rays, const double* intensit
                                int64_t num_t1, int64_t num_t2, int32_t coherent,
                                wq3d_eigenray_total* totals, double* total_intensity, double* total_phase);
 
+/* wq3d_sum_eigenrays without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device or host memory
+ * (tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors as wq3d_sum_eigenrays; the device memory of the call
+ * (the uploaded records, the per-target lists and the totals) lives in a
+ * 128 MiB array the library holds on each device for the life of the
+ * process, and a call whose records need more than that returns
+ * WQ3D_ERR_CAPACITY before any device work.  Calls on the same device share
+ * that array: calls that overlap on one device race. */
+wq3d_status wq3d_sum_eigenrays_noalloc(const wq3d_eigenray* rays, const double* intensity, const double* phase,
+                                       int64_t num_rays, const double* frequencies, int32_t num_freq,
+                                       int64_t num_t1, int64_t num_t2, int32_t coherent,
+                                       wq3d_eigenray_total* totals, double* total_intensity,
+                                       double* total_phase);
+
 /* Copy one queue slot to the host (synchronises the fan's stream). */
 wq3d_status wq3d_snapshot(wq3d_fan* fan, int32_t slot, wq3d_front_view* view);
 

@@ -58,6 +58,8 @@ def bind(lib):
         "wq3d_search_totals": ([vp, C.POINTER(i64), C.POINTER(i64)], i32),
         "wq3d_sum_eigenrays": ([C.c_void_p, dp, dp, i64, dp, i32, i64, i64, i32,
                                 C.POINTER(_abi.EigenrayTotal), dp, dp], i32),
+        "wq3d_sum_eigenrays_noalloc": ([C.c_void_p, dp, dp, i64, dp, i32, i64, i64, i32,
+                                        C.POINTER(_abi.EigenrayTotal), dp, dp], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)

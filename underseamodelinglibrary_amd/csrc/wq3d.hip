@@ -336,6 +336,58 @@ __global__ void __launch_bounds__(kBlock) k_sum_eigenrays(SumIn in, SumOut out, 
     if (t < targets) body_sum_eigenrays(t, in, out);
 }
 
+// Per-target lists of the allocation-free sum, built on the device in the
+// caller-less scratch array (see wq3d_sum_eigenrays_noalloc): clear, count
+// per target, scan to offsets, scatter ray indices, sort each list.
+__global__ void __launch_bounds__(kBlock) k_fill_i64(int64_t* p, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_count_targets(const wq3d_eigenray* rays, int64_t n, int64_t nt2,
+                                                          int64_t* offset) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) body_count_target(k, rays, nt2, offset);
+}
+
+// one workgroup: each thread scans a contiguous chunk, the chunk totals are
+// scanned in LDS, then each chunk adds its prefix
+__global__ void __launch_bounds__(kBlock) k_scan_offsets(int64_t* offset, int64_t n) {
+    __shared__ int64_t s_sum[kBlock];
+    const int64_t chunk = (n + kBlock - 1) / kBlock;
+    const int64_t b = (int64_t)threadIdx.x * chunk;
+    const int64_t e = (b + chunk < n) ? b + chunk : n;
+    int64_t acc = 0;
+    for (int64_t k = b; k < e; ++k) {
+        acc += offset[k];
+        offset[k] = acc;
+    }
+    s_sum[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (int k = 0; k < kBlock; ++k) {
+            const int64_t v = s_sum[k];
+            s_sum[k] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    const int64_t pre = s_sum[threadIdx.x];
+    for (int64_t k = b; k < e; ++k) offset[k] += pre;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter_rays(const wq3d_eigenray* rays, int64_t n, int64_t nt2,
+                                                         const int64_t* offset, int64_t* cursor, int64_t* order) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) body_scatter_ray(k, rays, nt2, offset, cursor, order);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sort_lists(const int64_t* offset, int64_t* order, int64_t targets) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < targets) body_sort_list(t, offset, order);
+}
+
 // is_closest_ray + build_eigenray per candidate, grid-stride over the
 // device-side candidate count; each candidate is built by a group of G
 // consecutive lanes, one per frequency (body_build<true>)
@@ -376,6 +428,8 @@ inline hipError_t event_elapsed(float* ms, hipEvent_t a, hipEvent_t b) { return 
 inline hipError_t event_destroy(hipEvent_t e) { return hipEventDestroy(e); }
 inline hipError_t event_sync(hipEvent_t e) { return hipEventSynchronize(e); }
 inline hipError_t last_error() { return hipGetLastError(); }
+template <class T>
+hipError_t symbol_address(void** p, const T& sym) { return hipGetSymbolAddress(p, HIP_SYMBOL(sym)); }
 #else
 // device allocations are not zeroed on the GPU: fill with 0xff (NaN doubles,
 // -1 integers) so reads before writes show up in the CPU checks.
@@ -439,6 +493,14 @@ inline hipError_t event_elapsed(float* ms, hipEvent_t, hipEvent_t) {
 inline hipError_t event_destroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t event_sync(hipEvent_t) { return hipSuccess; }
 inline hipError_t last_error() { return hipSuccess; }
+// the emulation's "device" copy of a file-scope array is the host object
+// itself, registered as a device allocation on first use
+template <class T>
+hipError_t symbol_address(void** p, const T& sym) {
+    *p = (void*)&sym;
+    emu::note_alloc(*p, sizeof(T));
+    return hipSuccess;
+}
 #endif
 }  // namespace rt
 
@@ -1746,6 +1808,151 @@ wq3d_status wq3d_sum_eigenrays(const wq3d_eigenray* rays, const double* intensit
     HIP_TRY(rt::memcpy(totals, d_tot, (size_t)targets * sizeof(wq3d_eigenray_total), hipMemcpyDeviceToHost));
     HIP_TRY(rt::memcpy(total_intensity, d_inten, (size_t)targets * num_freq * sizeof(double), hipMemcpyDeviceToHost));
     HIP_TRY(rt::memcpy(total_phase, d_phase, (size_t)targets * num_freq * sizeof(double), hipMemcpyDeviceToHost));
+    return WQ3D_OK;
+}
+
+}  // extern "C"
+
+// ---- allocation-free eigenray totals ------------------------------------
+// The device memory of wq3d_sum_eigenrays_noalloc: a file-scope array each
+// device holds until the process ends.  Host code never uses its name as an
+// address: scratch_address() looks the array up once per device
+// (hipGetSymbolAddress, with that device current) and every copy and kernel
+// argument takes that address.  Calls on one device share the array, so
+// calls that overlap on one device race.
+namespace {
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+#ifndef WQ3D_EMULATE
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+#else
+// the emulation build has no device code object: its "device" memory is
+// host memory, so the array is a host object
+alignas(256) unsigned char scratch[kNoAllocScratchBytes];
+#endif
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+wq3d_status scratch_address(int device, unsigned char** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(WQ3D_ERR_DEVICE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        HIP_TRY(rt::symbol_address(&p, scratch));
+        if (!p) return fail(WQ3D_ERR_DEVICE, "scratch array has no device address");
+        g_scratch_ptr[device] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_ptr[device]);
+    return WQ3D_OK;
+}
+
+// bump allocation of 256-byte aligned regions inside the scratch array
+struct ScratchLayout {
+    size_t used = 0;
+    size_t take(size_t bytes) {
+        const size_t at = used;
+        used += (bytes + 255) & ~(size_t)255;
+        return at;
+    }
+};
+}  // namespace
+
+extern "C" {
+
+wq3d_status wq3d_sum_eigenrays_noalloc(const wq3d_eigenray* rays, const double* intensity, const double* phase,
+                                       int64_t num_rays, const double* frequencies, int32_t num_freq,
+                                       int64_t num_t1, int64_t num_t2, int32_t coherent,
+                                       wq3d_eigenray_total* totals, double* total_intensity,
+                                       double* total_phase) {
+    if (num_rays < 0 || num_t1 < 0 || num_t2 < 0 || num_freq < 1)
+        return fail(WQ3D_ERR_ARGUMENT, "negative size or no frequency");
+    if ((num_rays > 0 && (!rays || !intensity || !phase)) || !frequencies ||
+        (num_t1 * num_t2 > 0 && (!totals || !total_intensity || !total_phase)))
+        return fail(WQ3D_ERR_ARGUMENT, "null argument");
+    const int64_t targets = num_t1 * num_t2;
+    if (targets == 0) return WQ3D_OK;
+    for (int64_t k = 0; k < num_rays; ++k)
+        if (rays[k].t1 < 0 || rays[k].t1 >= num_t1 || rays[k].t2 < 0 || rays[k].t2 >= num_t2)
+            return fail(WQ3D_ERR_ARGUMENT, "eigenray target outside the grid");
+    // the regions of one call, checked against the array's size before any
+    // device work
+    const size_t nr = (size_t)num_rays, nf = (size_t)num_freq, nt = (size_t)targets;
+    ScratchLayout L;
+    const size_t o_rays = L.take(nr * sizeof(wq3d_eigenray));
+    const size_t o_int = L.take(nr * nf * sizeof(double));
+    const size_t o_phase = L.take(nr * nf * sizeof(double));
+    const size_t o_freq = L.take(nf * sizeof(double));
+    const size_t o_offset = L.take((nt + 1) * sizeof(int64_t));
+    const size_t o_cursor = L.take(nt * sizeof(int64_t));
+    const size_t o_order = L.take(std::max<size_t>(1, nr) * sizeof(int64_t));
+    const size_t o_tot = L.take(nt * sizeof(wq3d_eigenray_total));
+    const size_t o_tint = L.take(nt * nf * sizeof(double));
+    const size_t o_tphase = L.take(nt * nf * sizeof(double));
+    if (L.used > kNoAllocScratchBytes)
+        return fail(WQ3D_ERR_CAPACITY, "eigenray sum needs " + std::to_string(L.used) +
+                                           " bytes of device memory; the allocation-free entry point has " +
+                                           std::to_string(kNoAllocScratchBytes));
+    if (wq3d_device_count() < 1) return fail(WQ3D_ERR_DEVICE, "no HIP device");
+    int device = 0;
+    HIP_TRY(rt::get_device(&device));
+    unsigned char* base = nullptr;
+    wq3d_status st = scratch_address(device, &base);
+    if (st) return st;
+    auto* d_rays = reinterpret_cast<wq3d_eigenray*>(base + o_rays);
+    auto* d_int = reinterpret_cast<double*>(base + o_int);
+    auto* d_phase = reinterpret_cast<double*>(base + o_phase);
+    auto* d_freq = reinterpret_cast<double*>(base + o_freq);
+    auto* d_offset = reinterpret_cast<int64_t*>(base + o_offset);
+    auto* d_cursor = reinterpret_cast<int64_t*>(base + o_cursor);
+    auto* d_order = reinterpret_cast<int64_t*>(base + o_order);
+    auto* d_tot = reinterpret_cast<wq3d_eigenray_total*>(base + o_tot);
+    auto* d_tint = reinterpret_cast<double*>(base + o_tint);
+    auto* d_tphase = reinterpret_cast<double*>(base + o_tphase);
+    if (nr) {
+        HIP_TRY(rt::memcpy(d_rays, rays, nr * sizeof(wq3d_eigenray), hipMemcpyHostToDevice));
+        HIP_TRY(rt::memcpy(d_int, intensity, nr * nf * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(rt::memcpy(d_phase, phase, nr * nf * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(rt::memcpy(d_freq, frequencies, nf * sizeof(double), hipMemcpyHostToDevice));
+    SumIn in{};
+    in.rays = d_rays;
+    in.inten = d_int;
+    in.phase = d_phase;
+    in.order = d_order;
+    in.offset = d_offset;
+    in.freq = d_freq;
+    in.nf = num_freq;
+    in.coherent = coherent != 0;
+    SumOut out{d_tot, d_tint, d_tphase};
+#ifndef WQ3D_EMULATE
+    auto blocks = [](int64_t n) { return dim3((unsigned)((std::max<int64_t>(1, n) + kBlock - 1) / kBlock)); };
+    // the array keeps the previous call's contents: the counts and cursors
+    // start from zero (a fill kernel, not a memset)
+    hipLaunchKernelGGL(k_fill_i64, blocks(targets + 1), dim3(kBlock), 0, 0, d_offset, targets + 1);
+    hipLaunchKernelGGL(k_fill_i64, blocks(targets), dim3(kBlock), 0, 0, d_cursor, targets);
+    hipLaunchKernelGGL(k_count_targets, blocks(num_rays), dim3(kBlock), 0, 0, d_rays, num_rays, num_t2, d_offset);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(kBlock), 0, 0, d_offset, targets + 1);
+    hipLaunchKernelGGL(k_scatter_rays, blocks(num_rays), dim3(kBlock), 0, 0, d_rays, num_rays, num_t2, d_offset,
+                       d_cursor, d_order);
+    hipLaunchKernelGGL(k_sort_lists, blocks(targets), dim3(kBlock), 0, 0, d_offset, d_order, targets);
+    hipLaunchKernelGGL(k_sum_eigenrays, blocks(targets), dim3(kBlock), 0, 0, in, out, targets);
+    HIP_TRY(rt::last_error());
+    HIP_TRY(hipDeviceSynchronize());
+#else
+    for (int64_t k = 0; k <= targets; ++k) d_offset[k] = 0;
+    for (int64_t k = 0; k < targets; ++k) d_cursor[k] = 0;
+    for (int64_t k = 0; k < num_rays; ++k) body_count_target(k, d_rays, num_t2, d_offset);
+    for (int64_t k = 1; k <= targets; ++k) d_offset[k] += d_offset[k - 1];
+    // scatter last to first: the per-target sort must restore arrival order
+    // whatever order the device's atomics hand out positions in
+    for (int64_t k = num_rays - 1; k >= 0; --k) body_scatter_ray(k, d_rays, num_t2, d_offset, d_cursor, d_order);
+    for (int64_t t = 0; t < targets; ++t) body_sort_list(t, d_offset, d_order);
+    for (int64_t t = 0; t < targets; ++t) body_sum_eigenrays(t, in, out);
+#endif
+    HIP_TRY(rt::memcpy(totals, d_tot, nt * sizeof(wq3d_eigenray_total), hipMemcpyDeviceToHost));
+    HIP_TRY(rt::memcpy(total_intensity, d_tint, nt * nf * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(rt::memcpy(total_phase, d_tphase, nt * nf * sizeof(double), hipMemcpyDeviceToHost));
     return WQ3D_OK;
 }
 

@@ -725,6 +725,37 @@ WQ_HD void body_sum_eigenrays(int64_t target, const SumIn& in, const SumOut& out
     T.count = (int32_t)(e - b);
 }
 
+// The per-target lists body_sum_eigenrays reads (order / offset), built on
+// the device: count per target into offset[t + 1], an inclusive scan turns
+// the counts into offsets, each ray takes a position in its target's list
+// with an atomic cursor, and each list is then sorted by ray index, which is
+// arrival order - the order a stable counting sort on the host gives.
+WQ_HD_INLINE void body_count_target(int64_t k, const wq3d_eigenray* rays, int64_t nt2, int64_t* offset) {
+    const int64_t t = (int64_t)rays[k].t1 * nt2 + rays[k].t2;
+    atom_add(reinterpret_cast<unsigned long long*>(offset + t + 1), 1ull);
+}
+
+WQ_HD_INLINE void body_scatter_ray(int64_t k, const wq3d_eigenray* rays, int64_t nt2, const int64_t* offset,
+                                   int64_t* cursor, int64_t* order) {
+    const int64_t t = (int64_t)rays[k].t1 * nt2 + rays[k].t2;
+    const unsigned long long at_ = atom_add(reinterpret_cast<unsigned long long*>(cursor + t), 1ull);
+    order[offset[t] + (int64_t)at_] = k;
+}
+
+// insertion sort of one target's list (lists hold tens of rays)
+WQ_HD_INLINE void body_sort_list(int64_t t, const int64_t* offset, int64_t* order) {
+    const int64_t b = offset[t], e = offset[t + 1];
+    for (int64_t k = b + 1; k < e; ++k) {
+        const int64_t v = order[k];
+        int64_t j = k;
+        while (j > b && order[j - 1] > v) {
+            order[j] = order[j - 1];
+            --j;
+        }
+        order[j] = v;
+    }
+}
+
 // --------------------------------------------------------------- advance
 // find_edges (gather form of wave_front.cc:184-216) on the old next slot,
 // then AB3 + update() of the new next slot and the carries of step()
