@@ -36,6 +36,23 @@ CASES = {
 }
 
 
+# The benchmark fan with all of its 10,000 targets for its first 35 steps
+# (config 2's fan, frequencies and targets in config 1's Munk ocean; about
+# 2.5 minutes of the oracle).  Heavy: the CPU suite re-runs it only with
+# WQ3D_SLOW=1; the device checks it on every GPU run.
+HEAVY_CASES = {
+    "config2_fan_munk_35": lambda: _steps(scenarios.config2_analytic(), 35),
+}
+
+
+def _steps(sc, n):
+    sc.steps = n
+    return sc
+
+
+CASES.update(HEAVY_CASES)
+
+
 def fixture_path(name):
     return os.path.join(HERE, f"{name}.npz")
 
@@ -49,8 +66,8 @@ def run_oracle(name):
     return sc, recs, inten, phase
 
 
-def main():
-    for name in CASES:
+def main(names=None):
+    for name in (names or CASES):
         t0 = time.time()
         sc, recs, inten, phase = run_oracle(name)
         np.savez_compressed(fixture_path(name), records=recs, intensity=inten, phase=phase,
@@ -60,4 +77,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or None)

@@ -2,7 +2,8 @@
 tests/golden/make_golden.py from the oracle).
 
 * CPU: the oracle still gives every fixture bit for bit (records, intensity,
-  phase), so a change to the oracle cannot pass unseen.
+  phase), so a change to the oracle cannot pass unseen.  The heavy fixture
+  (the benchmark fan with all its targets) only with WQ3D_SLOW=1.
 * GPU: the device, through the C ABI, gives each fixture's eigenray set to the
   parity bar (identical keys and path counters, travel time within 1e-9
   relative, intensity within 0.01 dB, angles within 1e-6 deg, phase within
@@ -22,6 +23,8 @@ make_golden = importlib.util.module_from_spec(_spec)
 _spec.loader.exec_module(make_golden)
 
 NAMES = sorted(make_golden.CASES)
+LIGHT = [n for n in NAMES if n not in make_golden.HEAVY_CASES]
+SLOW = os.environ.get("WQ3D_SLOW") == "1"
 
 
 def _load(name):
@@ -29,7 +32,7 @@ def _load(name):
         return {k: z[k] for k in z.files}
 
 
-@pytest.mark.parametrize("name", NAMES)
+@pytest.mark.parametrize("name", NAMES if SLOW else LIGHT)
 def test_oracle_reproduces_golden(name, oracle_mod):
     gold = _load(name)
     sc, recs, inten, phase = make_golden.run_oracle(name)

@@ -1,0 +1,218 @@
+"""The benchmark configurations on the device: config 5 in its own
+environment (64-node piecewise-linear c(z), flat 4500 m bottom) against the
+oracle and at full size, the benchmark fan's open AZ seam against the
+oracle, and a fan of config 3's size.
+
+Bar: identical eigenray keys (step, t1, t2, de, az) and counters, travel
+time within 1e-9 relative, intensity within 0.01 dB, angles within 1e-6 deg
+(tests/parity.py)."""
+import math
+
+import numpy as np
+import pytest
+
+from underseamodelinglibrary_amd import _abi, scenarios
+from underseamodelinglibrary_amd.usml.types import seq_linear, seq_log, wposition, wposition1
+
+from parity import compare_eigenrays, compare_fronts
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_run(sc, steps):
+    from underseamodelinglibrary_amd.usml.waveq3d import wave_queue
+    g = wave_queue(*sc.args())
+    g._lib.check(g._lib.wq3d_step(g._fan, steps))
+    return g
+
+
+def _oracle_run(oracle_mod, sc, steps):
+    o = oracle_mod.oracle_queue(*sc.args())
+    o._lib.orc_step(o._h, steps)
+    return o
+
+
+def _sane(recs, inten, phase, sc, steps, seam_az=()):
+    """Every record: target in the grid, finite positive travel time, finite
+    intensity and phase, counters non-negative, launch DE inside the fan.
+    Records of rays not next to an open AZ seam arrive within one step of
+    the step that found them."""
+    n1, n2 = sc.targets.size1(), sc.targets.size2()
+    assert (recs["t1"] >= 0).all() and (recs["t1"] < n1).all()
+    assert (recs["t2"] >= 0).all() and (recs["t2"] < n2).all()
+    assert np.isfinite(recs["travel_time"]).all() and (recs["travel_time"] > 0.0).all()
+    assert np.isfinite(inten).all() and np.isfinite(phase).all()
+    for name in ("surface", "bottom", "caustic", "upper", "lower"):
+        assert (recs[name] >= 0).all(), name
+    de0, de1 = sc.de(0), sc.de(sc.de.size() - 1)
+    assert (recs["source_de"] >= de0 - 0.5).all() and (recs["source_de"] <= de1 + 0.5).all()
+    away = ~np.isin(recs["az"], np.asarray(seam_az, dtype=np.int64))
+    dt = sc.time_step
+    late = np.abs(recs["travel_time"] - recs["step"] * dt) > dt
+    return int((late & away).sum()), int((late & ~away).sum())
+
+
+def test_config5_reduced_parity(oracle_mod):
+    """Config 5's environment (1-D linear profile grid of a perturbed Munk
+    profile, flat 4500 m Rayleigh-sand bottom, 500-4000 Hz) on a thinned fan
+    (DE -45:1:45, AZ 0:15:345) with 10 x 10 targets for 20 s: eigenrays and
+    all four wavefront slots against the oracle."""
+    sc = scenarios.config5(de=seq_linear(-45.0, 1.0, 45.0), az=seq_linear(0.0, 15.0, 345.0),
+                           n_targets=(10, 10), time_max=20.0)
+    print(f"[config5] reduced: {sc.num_rays()} rays, {sc.num_targets()} targets, {sc.steps} steps",
+          flush=True)
+    g = _gpu_run(sc, sc.steps)
+    o = _oracle_run(oracle_mod, sc, sc.steps)
+    gr, gi, gp = g.drain_eigenrays()
+    orr, oi, op = o.drain_eigenrays()
+    n = compare_eigenrays(gr, gi, gp, orr, oi, op, sc.freq.size())
+    for slot in (_abi.PAST, _abi.PREV, _abi.CURR, _abi.NEXT):
+        compare_fronts(g.snapshot_arrays(slot), o.snapshot_arrays(slot))
+    assert n > 100
+
+
+def test_config5_second_profile_parity(oracle_mod):
+    """Another of config 5's 32 seeded profiles, 5 frequencies (one more
+    than a construction group's minimum of 4 lanes), eigenverbs and
+    reflections attached, 15 s."""
+    sc = scenarios.config5(profile=7, de=seq_linear(-45.0, 1.5, 45.0), az=seq_linear(0.0, 20.0, 340.0),
+                           n_targets=(6, 6), time_max=15.0, freq=seq_log(500.0, 1.5, 5))
+    print(f"[config5] profile 7: {sc.num_rays()} rays, {sc.steps} steps", flush=True)
+    from underseamodelinglibrary_amd.usml.waveq3d import wave_queue
+    dummy = type("L", (), {"add_eigenverb": lambda *a: None, "reflect": lambda *a: None})()
+    g = wave_queue(*sc.args())
+    o = oracle_mod.oracle_queue(*sc.args())
+    for q in (g, o):
+        q.add_eigenverb_listener(dummy)
+        q.add_reflection_listener(dummy)
+    g._lib.check(g._lib.wq3d_step(g._fan, sc.steps))
+    o._lib.orc_step(o._h, sc.steps)
+    gr, gi, gp = g.drain_eigenrays()
+    orr, oi, op = o.drain_eigenrays()
+    assert compare_eigenrays(gr, gi, gp, orr, oi, op, sc.freq.size()) > 20
+    gv, gpw = g.drain_eigenverbs()
+    ov, opw = o.drain_eigenverbs()
+    assert len(gv) == len(ov) > 100
+    for k in ("step", "de", "az", "seq", "type", "surface", "bottom"):
+        np.testing.assert_array_equal(gv[k], ov[k], err_msg=k)
+    for k in ("travel_time", "length", "width", "grazing", "pos_rho", "pos_theta", "pos_phi"):
+        scale = np.maximum(np.abs(ov[k]), 1.0)
+        assert (np.abs(gv[k] - ov[k]) / scale).max() <= 1e-9, k
+
+
+def test_config5_full_run_against_oracle(oracle_mod):
+    """Config 5 at full size (901 x 72 rays, 4 frequencies, 50 x 50 targets,
+    2401 steps of 0.05 s).  Every eigenray record is sane.  Records more than
+    one step away from the step that found them (compute_offsets
+    extrapolating outside the ray family: next to the open AZ seam, AZ 355 ->
+    0 is not a wrap, at the DE edges of the fan and on unstable
+    neighbourhoods) are checked against the oracle: the 12 targets with the
+    most such records, rerun on the full fan for all 2401 steps on the
+    device and in the oracle, give identical eigenray sets to the parity
+    bar, and the device's full run holds exactly the same records for them."""
+    sc = scenarios.config5()
+    print(f"[config5] full: {sc.num_rays()} rays, {sc.num_targets()} targets, {sc.steps} steps",
+          flush=True)
+    g = _gpu_run(sc, sc.steps)
+    recs, inten, phase = g.drain_eigenrays()
+    g.close()
+    print(f"[config5] {len(recs)} eigenrays", flush=True)
+    assert len(recs) > 10000
+    late_away, late_seam = _sane(recs, inten, phase, sc, sc.steps, seam_az=(1, sc.az.size() - 2))
+    late = np.abs(recs["travel_time"] - recs["step"] * sc.time_step) > sc.time_step
+    nde = sc.de.size()
+    edge_de = np.isin(recs["de"], [1, nde - 2])
+    print(f"[config5] late records: {late_away} away from the AZ seam ({int((late & edge_de).sum())} at the "
+          f"DE edges), {late_seam} beside it", flush=True)
+    n2 = sc.targets.size2()
+    tid = recs["t1"].astype(np.int64) * n2 + recs["t2"]
+    counts = np.bincount(tid[late], minlength=sc.num_targets())
+    pick = np.argsort(-counts, kind="stable")[:12]
+    pick = np.sort(pick[counts[pick] > 0])
+    assert pick.size > 0
+    sub = scenarios.config5()
+    rho = sc.targets.rho.reshape(-1)[pick]
+    theta = sc.targets.theta.reshape(-1)[pick]
+    phi = sc.targets.phi.reshape(-1)[pick]
+    sub.targets = wposition(1, pick.size, 0.0, 0.0, 0.0)
+    sub.targets.rho[0, :], sub.targets.theta[0, :], sub.targets.phi[0, :] = rho, theta, phi
+    print(f"[config5] oracle on the full fan, {pick.size} targets, {sub.steps} steps", flush=True)
+    gs = _gpu_run(sub, sub.steps)
+    os_ = _oracle_run(oracle_mod, sub, sub.steps)
+    gr, gi, gp = gs.drain_eigenrays()
+    orr, oi, op = os_.drain_eigenrays()
+    n = compare_eigenrays(gr, gi, gp, orr, oi, op, sub.freq.size())
+    sub_late = int((np.abs(gr["travel_time"] - gr["step"] * sub.time_step) > sub.time_step).sum())
+    print(f"[config5] subset: {n} eigenrays, {sub_late} late, identical to the oracle's", flush=True)
+    assert sub_late > 0
+    # the full run's records of those targets are the subset run's
+    nf = sc.freq.size()
+    keep = np.isin(tid, pick)
+    full = recs[keep]
+    remap = {int(t): k for k, t in enumerate(pick)}
+    full_t2 = np.array([remap[int(t)] for t in tid[keep]])
+    assert len(full) == len(gr)
+    order = np.lexsort((full["az"], full["de"], full_t2, full["step"]))
+    for name in ("step", "de", "az", "travel_time", "source_de", "source_az", "surface", "bottom", "caustic"):
+        np.testing.assert_array_equal(full[name][order], gr[name], err_msg=name)
+    np.testing.assert_array_equal(full_t2[order], gr["t2"])
+    fi = inten.reshape(-1, nf)[keep][order].reshape(-1)
+    np.testing.assert_array_equal(fi, gi)
+
+
+def _seam_targets(src, bearings_deg, ranges_m, depths_m):
+    lat = np.empty((len(bearings_deg), len(ranges_m)))
+    lng = np.empty_like(lat)
+    alt = np.empty_like(lat)
+    for a, b in enumerate(bearings_deg):
+        for r, rng in enumerate(ranges_m):
+            p = wposition1.from_range_bearing(src, rng, math.radians(b))
+            lat[a, r], lng[a, r] = p.latitude(), p.longitude()
+            alt[a, r] = -depths_m[r]
+    return wposition.from_arrays(lat, lng, alt)
+
+
+def test_open_az_seam_full_fan_parity(oracle_mod):
+    """The benchmark's full fan (DE -80:0.25:80, AZ 0:2:358, 8 frequencies,
+    Munk ocean) for 150 steps with targets in and beside the open AZ seam
+    (bearings 357-1 deg, 2-12 km): the records next to the seam, where
+    compute_offsets extrapolates outside the ray family
+    (wave_queue.cc:437-442), are the oracle's restatement of the reference's
+    to the parity bar."""
+    sc = scenarios.config2_analytic(n_targets=(1, 1))
+    sc.targets = _seam_targets(sc.pos, [357.0, 358.5, 359.0, 359.5, 0.0, 1.0], [2000.0, 5000.0, 8000.0, 12000.0],
+                               [200.0, 800.0, 1200.0, 1800.0])
+    steps = 150
+    print(f"[seam] {sc.num_rays()} rays, {sc.num_targets()} seam targets, {steps} steps", flush=True)
+    g = _gpu_run(sc, steps)
+    o = _oracle_run(oracle_mod, sc, steps)
+    gr, gi, gp = g.drain_eigenrays()
+    orr, oi, op = o.drain_eigenrays()
+    n = compare_eigenrays(gr, gi, gp, orr, oi, op, sc.freq.size())
+    seam = np.isin(gr["az"], [1, sc.az.size() - 2])
+    late = np.abs(gr["travel_time"] - gr["step"] * sc.time_step) > sc.time_step
+    print(f"[seam] {n} eigenrays, {int(seam.sum())} next to the seam, {int((late & seam).sum())} of them "
+          f"extrapolated beyond a step, {int((late & ~seam).sum())} elsewhere", flush=True)
+    assert n > 50
+
+
+def test_config3_size_fan_sane():
+    """A fan of config 3's size (DE -89:0.1:89 x AZ 0:1:359 = 641,160 rays,
+    16 frequencies, 200 x 200 targets) in config 1's Munk ocean for 40 steps:
+    the device holds it and every record is sane; two runs give identical
+    records."""
+    sc = scenarios.config1(de=seq_linear(-89.0, 0.1, 89.0), az=seq_linear(0.0, 1.0, 359.0),
+                           n_targets=(200, 200), time_max=300.0)
+    sc.freq = seq_log(100.0, 10.0 ** (2.0 / 15.0), 16)
+    steps = 40
+    print(f"[config3 size] {sc.num_rays()} rays, {sc.num_targets()} targets, {steps} steps", flush=True)
+    runs = []
+    for _ in range(2):
+        g = _gpu_run(sc, steps)
+        runs.append(g.drain_eigenrays())
+        g.close()
+    (r0, i0, p0), (r1, i1, p1) = runs
+    assert len(r0) == len(r1) > 100
+    assert r0.tobytes() == r1.tobytes() and i0.tobytes() == i1.tobytes()
+    late_away, _ = _sane(r0, i0, p0, sc, steps, seam_az=(1, sc.az.size() - 2))
+    assert late_away == 0

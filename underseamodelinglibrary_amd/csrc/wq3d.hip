@@ -616,6 +616,7 @@ int env_variant(const DEnv& E) {
     const bool flat = E.surface.kind == WQ3D_BOUNDARY_FLAT && E.bottom.kind == WQ3D_BOUNDARY_FLAT;
     if (flat && E.profile.kind == WQ3D_PROFILE_MUNK) return 1;
     if (flat && E.profile.kind == WQ3D_PROFILE_LINEAR) return 2;
+    if (flat && is_grid1_linear(E.profile)) return 3;
     return 0;
 }
 #define WQ3D_LAUNCH_ENV(kernel, E, grid, st, ...)                                              \
@@ -626,6 +627,9 @@ int env_variant(const DEnv& E) {
             break;                                                                             \
         case 2:                                                                                \
             hipLaunchKernelGGL((kernel<WQ3D_PROFILE_LINEAR, true>), dim3(grid), dim3(kBlock), 0, st, E, __VA_ARGS__); \
+            break;                                                                             \
+        case 3:                                                                                \
+            hipLaunchKernelGGL((kernel<kProfileGrid1Linear, true>), dim3(grid), dim3(kBlock), 0, st, E, __VA_ARGS__); \
             break;                                                                             \
         default:                                                                               \
             hipLaunchKernelGGL((kernel<-1, false>), dim3(grid), dim3(kBlock), 0, st, E, __VA_ARGS__); \
@@ -652,6 +656,9 @@ void launch_bounce(hipStream_t st, unsigned grid, const KParams* kp, const DEnv&
         break;
     case 2:
         hipLaunchKernelGGL((k_bounce<WQ3D_PROFILE_LINEAR, true>), dim3(grid), dim3(kBlock), 0, st, kp, S, t, key);
+        break;
+    case 3:
+        hipLaunchKernelGGL((k_bounce<kProfileGrid1Linear, true>), dim3(grid), dim3(kBlock), 0, st, kp, S, t, key);
         break;
     default:
         hipLaunchKernelGGL((k_bounce<-1, false>), dim3(grid), dim3(kBlock), 0, st, kp, S, t, key);
@@ -1208,10 +1215,18 @@ wq3d_status wq3d_env_create(const wq3d_env_desc* desc, wq3d_env** out) {
     // grid lookups and table reads, so its cause is not yet shown.  The same
     // kernel bodies run clean and bit-identical to the oracle in the
     // bounds-checked CPU emulation build.
-    if (desc->profile.kind == WQ3D_PROFILE_GRID || desc->bottom.kind == WQ3D_BOUNDARY_GRID)
+    // The exception is a 1-D sound speed grid with linear interpolation over
+    // flat boundaries: it has its own kernel variant with the interpolation
+    // inline, as the analytic profiles have, and runs none of the faulting
+    // case's own code (the recursive 3-D / 2-D grid interpolation, the
+    // sloped-bottom reflections, the generic kernel variants).
+    const bool grid1_flat = desc->profile.kind == WQ3D_PROFILE_GRID && desc->profile.grid.ndims == 1 &&
+                            desc->profile.grid.interp[0] == WQ3D_INTERP_LINEAR &&
+                            desc->surface.kind == WQ3D_BOUNDARY_FLAT && desc->bottom.kind == WQ3D_BOUNDARY_FLAT;
+    if ((desc->profile.kind == WQ3D_PROFILE_GRID && !grid1_flat) || desc->bottom.kind == WQ3D_BOUNDARY_GRID)
         return fail(WQ3D_ERR_UNSUPPORTED,
-                    "gridded sound speed / bathymetry are disabled on the GPU "
-                    "(device fault with cause not yet shown; CPU-checked only)");
+                    "gridded sound speed / bathymetry are disabled on the GPU except a 1-D linear sound "
+                    "speed grid over flat boundaries (device fault with cause not yet shown; CPU-checked only)");
 #endif
     auto* env = new wq3d_env();
     HIP_TRY(rt::get_device(&env->device));

@@ -334,6 +334,47 @@ __host__ __device__ double grid_interpolate(const DGrid& g, const double locatio
     return grid_interp<ND - 1>(g, idx, loc, dresult, dv, want);
 }
 
+// profile_grid<1> with linear interpolation in rho: gen_grid::interpolate's
+// edge limit and index search (types/gen_grid.h:83-133) and the linear
+// level (:248-283), the same operations as grid_interpolate<1> with
+// grid_level<0>, written inline.  The index is in [0, n-2] for every input.
+__host__ __device__ __forceinline__ double grid1_linear(const DGrid& g, double rho, double& grho, bool want) {
+    const DSeq& a = g.ax[0];
+    double loc = rho;
+    int k;
+    if (g.edge[0]) {
+        const double lo = a.v[0];
+        const double hi = a.v[a.n - 1];
+        const double sgn = (a.inc[0] < 0) ? -1.0 : 1.0;
+        const double x = loc * sgn;
+        if (x <= lo * sgn) {
+            loc = lo;
+            k = 0;
+        } else if (x >= hi * sgn) {
+            loc = hi;
+            k = a.n - 2;
+        } else {
+            k = find_index(a, loc);
+        }
+    } else {
+        k = find_index(a, loc);
+    }
+    const double va = g.data[k];
+    const double vb = g.data[k + 1];
+    const double h = seq_inc(a, k);
+    const double u = (loc - seq_value(a, k)) / h;
+    if (want) grho = (vb - va) / h;
+    return va * (1.0 - u) + vb * u;
+}
+
+// profile kind of a kernel variant for a 1-D linear profile grid (not an
+// ABI value: the descriptor says WQ3D_PROFILE_GRID)
+constexpr int kProfileGrid1Linear = 3;
+
+__host__ __device__ __forceinline__ bool is_grid1_linear(const DProfile& P) {
+    return P.kind == 2 && P.grid.nd == 1 && P.grid.interp[0] == 1;
+}
+
 // Environment kinds known at compile time.  The functions on the
 // propagation and reflection path take a policy K: with AnyKinds they read
 // the kinds from the descriptor; with Kinds<PK, FLAT> the profile kind is PK
@@ -371,6 +412,8 @@ __host__ __device__ __forceinline__ void sound_speed(const DProfile& P, double r
             speed = P.p[0] + P.p[1] * P.p[2] + P.p[3] * (z - P.p[2]);
             grho = -P.p[3];
         }
+    } else if (kind == kProfileGrid1Linear || is_grid1_linear(P)) {
+        speed = grid1_linear(P.grid, rho, grho, want_gradient);
     } else if (P.grid.nd == 1) {
         double loc[1] = {rho};
         double dv[3] = {0.0, 0.0, 0.0};

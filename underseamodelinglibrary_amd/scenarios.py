@@ -8,6 +8,10 @@ produces with `time += time_step` accumulation (wave_queue.cc:167).
            flat 5000 m Rayleigh-sand bottom, flat surface, Thorp; source
            45N 45W 1000 m; DE -60:1:60, AZ 0:10:350, 1000 Hz; dt 0.1 s,
            t < 60 s; 10 x 10 targets within 0.5 deg, 100-2000 m deep.
+  config5  range-independent: 64-node piecewise-linear c(z) (1-D
+           profile_grid, linear in rho) from seeded perturbed Munk
+           profiles, flat 4500 m bottom; DE -45:0.1:45, AZ 0:5:355,
+           500-4000 Hz, dt 0.05 s, t < 120 s, 50 x 50 targets.
   config2  range-dependent: 3-D sound speed grid (1/4 deg over a 10 x 10 deg
            box, 40 depths to 5000 m) from Mackenzie on seeded T/S fields;
            1/60 deg bathymetry from 50 seeded Gaussian seamounts/troughs on
@@ -169,6 +173,58 @@ def config2(de=None, az=None, n_targets=(100, 100), time_max=120.0, freq=None, s
     tg = (target_grid(45.0, -45.0, 0.5, n_targets[0], n_targets[1], 100.0, 2000.0)
           if n_targets else None)
     return scenario("config2", ocean, freq, pos, de, az, 0.1, tg, loop_steps(0.1, time_max))
+
+
+def config5_profile(profile=0, n_nodes=64, bottom_depth=4500.0, sigma=2.0):
+    """Sound speed nodes of config 5's perturbed Munk profiles: n_nodes depths
+    evenly spaced from the surface to the bottom, the Munk profile (axis
+    1300 m, scale 1300 m, 1500 m/s, eps 0.00737) plus Gaussian noise of
+    standard deviation `sigma` m/s smoothed over about three nodes and
+    rescaled to that RMS; `profile` seeds the noise.  Returns (depths,
+    speeds)."""
+    depths = np.linspace(0.0, bottom_depth, n_nodes)
+    z = 2.0 * (depths - 1300.0) / 1300.0
+    munk = 1500.0 * (1.0 + 0.00737 * (z - 1.0 + np.exp(-z)))
+    rng = np.random.default_rng(5000 + int(profile))
+    noise = rng.normal(0.0, 1.0, n_nodes + 12)
+    kernel = np.exp(-0.5 * (np.arange(-6, 7) / 2.0) ** 2)
+    smooth = np.convolve(noise, kernel / kernel.sum(), mode="valid")[:n_nodes]
+    smooth *= sigma / math.sqrt(float(np.mean(smooth * smooth)))
+    return depths, munk + smooth
+
+
+def config5_ocean(speeds, bottom_depth=4500.0):
+    """Config 5's ocean for given node sound speeds: a 1-D profile_grid
+    linear in rho (axis rho = R - depth, edge limits on), Thorp attenuation,
+    flat bottom at bottom_depth with Rayleigh sand, flat surface."""
+    R = earth_radius()
+    n = len(speeds)
+    rho_axis = seq_linear(R, -bottom_depth / (n - 1), n)
+    grid = oc.data_grid([rho_axis], np.asarray(speeds, dtype=np.float64), [oc.LINEAR], [True])
+    profile = oc.profile_grid(grid, oc.attenuation_thorp())
+    bottom = oc.boundary_flat(bottom_depth, oc.reflect_loss_rayleigh("sand"))
+    return oc.ocean_model(oc.boundary_flat(), bottom, profile)
+
+
+def config5(profile=0, de=None, az=None, n_targets=(50, 50), time_max=120.0, freq=None, speeds=None):
+    """Config 5's forward run for one of its 32 profiles: 64-node piecewise
+    linear c(z) (seeded Munk plus 2 m/s smoothed noise), range-independent,
+    flat 4500 m bottom; DE -45:0.1:45 (901), AZ 0:5:355 (72), 500-4000 Hz
+    (4 octave-spaced frequencies), dt 0.05 s, t < 120 s, 50 x 50 targets
+    within 0.5 deg of the source (45N 45W, 1000 m) at 100-2000 m depth."""
+    set_earth_radius(6371000.0)
+    if speeds is None:
+        speeds = config5_profile(profile)[1]
+    ocean = config5_ocean(speeds)
+    freq = freq if freq is not None else seq_log(500.0, 2.0, 4)
+    pos = wposition1(45.0, -45.0, -1000.0)
+    de = de if de is not None else seq_linear(-45.0, 0.1, 45.0)
+    az = az if az is not None else seq_linear(0.0, 5.0, 355.0)
+    tg = (target_grid(45.0, -45.0, 0.5, n_targets[0], n_targets[1], 100.0, 2000.0)
+          if n_targets else None)
+    sc = scenario("config5", ocean, freq, pos, de, az, 0.05, tg, loop_steps(0.05, time_max))
+    sc.notes["profile"] = profile
+    return sc
 
 
 def config2_analytic(n_targets=(100, 100), time_max=120.0):
