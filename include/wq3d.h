@@ -249,6 +249,39 @@ wq3d_status wq3d_fan_destroy(wq3d_fan* fan);
 /* Advance n steps of wave_queue::step().  Asynchronous on the fan's stream. */
 wq3d_status wq3d_step(wq3d_fan* fan, int64_t n);
 
+/* ---------------------------------------------------------------- batches */
+/* Many fans stepped together.  Replaces the reference's pool of
+ * wavefront_generator tasks, each stepping its own wave_queue
+ * (wavegen/wavefront_generator.cc:52-127 on threads/thread_pool.cc:21-75):
+ * wq3d_batch_step(batch, n) is n calls of step() on every fan of the batch,
+ * with one fused launch per kernel of a step for all fans (each fan owns a
+ * range of the launch's blocks), 64-step loops replayed as HIP graphs, and
+ * the records of each fan kept and drained per fan as wq3d_step keeps them
+ * (wq3d_drain_* on the fan).  The fans may differ in size, frequencies,
+ * targets and environment; they must be on one device and at the same step
+ * (fresh fans are), and are stepped only through the batch while it exists.
+ * wq3d_batch_step returns when the n steps are done. */
+typedef struct wq3d_batch wq3d_batch;
+
+/* Launch shape of a batch's fused launches (results never depend on it). */
+typedef struct wq3d_batch_shape {
+    int32_t ray_threads;   /* threads per block of the per-ray launches: 64, 128 or 256 */
+    int32_t build_blocks;  /* blocks of the eigenray-construction launch */
+    int32_t use_graph;     /* 1: 64-step loops replayed as a captured HIP graph */
+    int32_t pad_;
+} wq3d_batch_shape;
+
+wq3d_status wq3d_batch_create(wq3d_fan* const* fans, int32_t num_fans, wq3d_batch** batch);
+wq3d_status wq3d_batch_destroy(wq3d_batch* batch);
+wq3d_status wq3d_batch_step(wq3d_batch* batch, int64_t n);
+wq3d_status wq3d_batch_get_shape(const wq3d_batch* batch, wq3d_batch_shape* shape);
+wq3d_status wq3d_batch_set_shape(wq3d_batch* batch, const wq3d_batch_shape* shape);
+/* The autotuner: times `trial` real steps of the batch for each candidate
+ * shape (threads per block, then construction blocks), keeps the fastest and
+ * returns its time per step.  Advances the batch by the steps it times plus
+ * one per candidate; steps_taken returns that count. */
+wq3d_status wq3d_batch_tune(wq3d_batch* batch, int32_t trial, double* best_ms_per_step, int64_t* steps_taken);
+
 /* wave_thresholds setters (waveq3d/wave_thresholds.h:54-192) and listener
  * presence (eigenverb_notifier::has_eigenverb_listeners,
  * reflection_notifier::has_reflection_listeners); both take effect at the

@@ -40,6 +40,7 @@ STRUCTS = {
     "wq3d_fan_desc": _abi.FanDesc, "wq3d_eigenray": _abi.EigenrayRec,
     "wq3d_eigenverb": _abi.EigenverbRec, "wq3d_reflection": _abi.ReflectionRec,
     "wq3d_front_view": _abi.FrontView, "wq3d_eigenray_total": _abi.EigenrayTotal,
+    "wq3d_batch_shape": _abi.BatchShape,
 }
 
 

@@ -1,0 +1,152 @@
+"""Batches of fans (wq3d_batch_*): uneven fans of different sizes,
+frequency counts, environments (Munk, a 1-D linear sound speed grid, a
+linear profile), with and without targets and listeners, stepped together
+through the fused launches (and HIP graphs on the device) give every fan
+exactly the records and wavefronts it gets when stepped alone.  The CPU case
+runs the emulation build's batch driver under AddressSanitizer; the GPU case
+runs the fused kernels, with graphs, without them, and after the autotuner."""
+import numpy as np
+import pytest
+
+import emulation
+from underseamodelinglibrary_amd import _abi, scenarios
+from underseamodelinglibrary_amd.usml.types import seq_linear, seq_log
+
+STEPS = 150  # crosses two record-set rotations (64-step graphs on the device)
+
+
+def _cases():
+    """(name, scenario factory, listeners): each queue is built right after
+    its scenario, since scenarios set the global earth radius the queue
+    reads."""
+    def a():
+        return scenarios.config1(de=seq_linear(-60.0, 3.0, 60.0), az=seq_linear(0.0, 30.0, 330.0),
+                                 n_targets=(4, 4))
+
+    def b():
+        sc = scenarios.config1(de=seq_linear(-30.0, 2.0, 30.0), az=seq_linear(0.0, 45.0, 315.0), targets=False)
+        sc.freq = seq_log(500.0, 2.0, 3)
+        return sc
+
+    def c():
+        sc = scenarios.config5(profile=3, de=seq_linear(-45.0, 2.0, 45.0), az=seq_linear(0.0, 20.0, 340.0),
+                               n_targets=(5, 5))
+        sc.targets = scenarios.target_grid(45.0, -45.0, 0.05, 3, 3, 200.0, 1800.0)
+        return sc
+
+    def e():
+        return scenarios.config1(de=seq_linear(-10.0, 1.0, 1), az=seq_linear(0.0, 1.0, 1), targets=False)
+
+    return [("munk_targets_verbs", a, True), ("munk_no_targets", b, False), ("grid1_targets", c, False),
+            ("linear_basic", scenarios.eigenray_basic, False), ("single_ray", e, False)]
+
+
+def _queue(sc, lib, verbs):
+    from underseamodelinglibrary_amd.usml.waveq3d import wave_queue
+    q = wave_queue(*sc.args(), lib=lib)
+    if verbs:
+        dummy = type("L", (), {"add_eigenverb": lambda *a: None, "reflect": lambda *a: None})()
+        q._dummy = dummy
+        q.add_eigenverb_listener(dummy)
+        q.add_reflection_listener(dummy)
+    return q
+
+
+def _records(q):
+    out = list(q.drain_eigenrays()) + list(q.drain_eigenverbs()) + [q.drain_reflections()]
+    for slot in (_abi.PAST, _abi.PREV, _abi.CURR, _abi.NEXT):
+        out += [v for _, v in sorted(q.snapshot_arrays(slot).items())]
+    return out
+
+
+def run_batch_vs_alone(lib, mode):
+    """mode: 'graph', 'eager' or 'tuned'.  Returns the per-fan record counts
+    after asserting every fan's records equal its run alone."""
+    from underseamodelinglibrary_amd.usml.waveq3d import fan_batch
+    cases = _cases()
+    alone = []
+    for name, make, verbs in cases:
+        q = _queue(make(), lib, verbs)
+        lib.check(lib.wq3d_step(q._fan, STEPS))
+        alone.append(_records(q))
+        q.close()
+    queues = [_queue(make(), lib, verbs) for _, make, verbs in cases]
+    batch = fan_batch(queues, lib=lib)
+    done = 0
+    if mode == "eager":
+        batch.set_shape(128, 512, use_graph=False)
+    elif mode == "tuned":
+        # the autotuner through the C ABI (its steps' records stay queued)
+        import ctypes as C
+        ms, taken = C.c_double(), C.c_int64()
+        lib.check(lib.wq3d_batch_tune(batch._h, 3, C.byref(ms), C.byref(taken)))
+        assert batch.shape()["ray_threads"] in (64, 128, 256) and ms.value > 0.0
+        done = taken.value
+        assert 0 < done < STEPS
+    # raw stepping, records compared raw (no listener replay)
+    lib.check(lib.wq3d_batch_step(batch._h, STEPS - done))
+    counts = []
+    for (name, _, _), q, ref in zip(cases, queues, alone):
+        got = _records(q)
+        assert len(got) == len(ref)
+        for k, (x, y) in enumerate(zip(got, ref)):
+            assert x.dtype == y.dtype and x.shape == y.shape, (name, k)
+            assert x.tobytes() == y.tobytes(), (name, k)
+        counts.append(len(ref[0]))
+    batch.close()
+    for q in queues:
+        q.close()
+    return counts
+
+
+def test_batch_equals_each_alone_emulated():
+    res = emulation.run_script_under_asan(
+        "import emulation, test_batch as t\nlib = emulation.load()\n"
+        "print('counts', *t.run_batch_vs_alone(lib, 'eager'))\n", timeout=900)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-4000:]
+    counts = [int(x) for x in res.stdout.split("counts")[-1].split()]
+    assert counts[0] > 0 and counts[2] > 0 and counts[3] >= 3
+
+
+def test_batch_rejects_misuse_emulated():
+    """Duplicate fans, fans at different steps, a fan stepped alone after
+    joining a batch: each a clear error, no step taken."""
+    res = emulation.run_script_under_asan("""
+import ctypes as C, emulation, test_batch as t
+from underseamodelinglibrary_amd import _abi, scenarios
+lib = emulation.load()
+cases = t._cases()
+q0 = t._queue(cases[0][1](), lib, False)
+q1 = t._queue(cases[1][1](), lib, False)
+arr = (C.c_void_p * 2)(q0._fan.value, q0._fan.value)
+h = C.c_void_p()
+print('dup', lib.wq3d_batch_create(arr, 2, C.byref(h)))
+lib.check(lib.wq3d_step(q1._fan, 3))
+arr = (C.c_void_p * 2)(q0._fan.value, q1._fan.value)
+print('apart', lib.wq3d_batch_create(arr, 2, C.byref(h)))
+q2 = t._queue(cases[1][1](), lib, False)
+arr = (C.c_void_p * 2)(q0._fan.value, q2._fan.value)
+lib.check(lib.wq3d_batch_create(arr, 2, C.byref(h)))
+lib.check(lib.wq3d_batch_step(h, 2))
+lib.check(lib.wq3d_step(q2._fan, 1))
+print('alone', lib.wq3d_batch_step(h, 1), lib.wq3d_last_error().decode())
+print('shape', lib.wq3d_batch_set_shape(h, C.byref(_abi.BatchShape(96, 512, 1, 0))))
+lib.wq3d_batch_destroy(h)
+""", timeout=600)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-4000:]
+    lines = dict(l.split(" ", 1) for l in res.stdout.strip().splitlines() if " " in l)
+    assert lines["dup"] == str(_abi.WQ3D_ERR_ARGUMENT)
+    assert lines["apart"] == str(_abi.WQ3D_ERR_STATE)
+    assert lines["alone"].startswith(str(_abi.WQ3D_ERR_STATE)) and "same step" in lines["alone"]
+    assert lines["shape"] == str(_abi.WQ3D_ERR_ARGUMENT)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["graph", "eager", "tuned"])
+def test_batch_equals_each_alone_gpu(mode):
+    from underseamodelinglibrary_amd import _lib
+    lib = _lib.load()
+    print(f"[batch] {mode}: {len(_cases())} uneven fans, {STEPS} steps", flush=True)
+    counts = run_batch_vs_alone(lib, mode)
+    print(f"[batch] eigenrays per fan {counts}", flush=True)
+    assert counts[0] > 0 and counts[2] > 0 and counts[3] >= 3

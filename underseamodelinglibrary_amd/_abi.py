@@ -57,6 +57,11 @@ class EnvDesc(C.Structure):
                 ("volume", C.POINTER(VolumeDesc))]
 
 
+class BatchShape(C.Structure):
+    _fields_ = [("ray_threads", C.c_int32), ("build_blocks", C.c_int32), ("use_graph", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
 class Thresholds(C.Structure):
     _fields_ = [("intensity", C.c_double), ("eigenverb", C.c_double),
                 ("max_bottom", C.c_int32), ("max_surface", C.c_int32),

@@ -60,6 +60,12 @@ def bind(lib):
                                 C.POINTER(_abi.EigenrayTotal), dp, dp], i32),
         "wq3d_sum_eigenrays_noalloc": ([C.c_void_p, dp, dp, i64, dp, i32, i64, i64, i32,
                                         C.POINTER(_abi.EigenrayTotal), dp, dp], i32),
+        "wq3d_batch_create": ([C.POINTER(vp), i32, C.POINTER(vp)], i32),
+        "wq3d_batch_destroy": ([vp], i32),
+        "wq3d_batch_step": ([vp, i64], i32),
+        "wq3d_batch_get_shape": ([vp, C.POINTER(_abi.BatchShape)], i32),
+        "wq3d_batch_set_shape": ([vp, C.POINTER(_abi.BatchShape)], i32),
+        "wq3d_batch_tune": ([vp, i32, dp, C.POINTER(i64)], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)

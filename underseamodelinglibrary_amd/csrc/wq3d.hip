@@ -51,6 +51,7 @@ using namespace wq3d;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kMaxBatchFans = 1024;  // fans of one batch (wq3d_batch_create)
 constexpr int kBuildLanes = 32;  // lanes per eigenray candidate in k_build
 // k_detect: ray tiles of kTile x kTile (AZ x DE, the tiles of the window
 // records), one ray per lane, and hit keys buffered per wavefront before a
@@ -70,6 +71,14 @@ struct KParams {
     FanConst F;
     Arrays A;
     Records R;
+};
+
+// a fan's time and step count in device memory (fused batch launches)
+struct FanClock {
+    double time;   // wave_queue::time()
+    int64_t step;  // steps taken
+    double dt;
+    int64_t pad;
 };
 
 #ifndef WQ3D_EMULATE
@@ -121,11 +130,12 @@ __global__ void __launch_bounds__(kBlock) k_advance(DEnv E, FanConst F, Arrays A
 // passes super-block u's window bound, so k_detect reads no record of a
 // tile for a super-block whose bit is clear.  One wavefront per tile; it
 // also counts the (ray, super-block) bounds tested.
-__global__ void __launch_bounds__(kBlock) k_supmask(FanConst F, Arrays A, Records Rc, int64_t tiles) {
+__device__ __forceinline__ void supmask_block(const FanConst& F, const Arrays& A, const Records& Rc, int64_t tiles,
+                                              int64_t blk) {
     // super-block boxes staged in LDS 32 at a time (one mask word)
     __shared__ TargetBox s_sb[32];
     const int lane = threadIdx.x & 63;
-    const int64_t tile = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int64_t tile = blk * (kBlock / 64) + (threadIdx.x >> 6);
     const bool have_tile = tile < tiles;  // wave-uniform; every thread stays for the barriers
     const int64_t tiles_az = (F.naz + kTile - 1) / kTile;
     const int64_t de = (tile / tiles_az) * kTile + lane / kTile;
@@ -148,7 +158,11 @@ __global__ void __launch_bounds__(kBlock) k_supmask(FanConst F, Arrays A, Record
         if (have_tile && lane == 0) A.tsup[tile * words + w0] = bits;
     }
     if (lane == 0 && n_searched)
-        atomicAdd(&Rc.scan[(blockIdx.x % kScanSlots) * 2 + 1], n_searched * (unsigned long long)F.nsup);
+        atomicAdd(&Rc.scan[(blk % kScanSlots) * 2 + 1], n_searched * (unsigned long long)F.nsup);
+}
+
+__global__ void __launch_bounds__(kBlock) k_supmask(FanConst F, Arrays A, Records Rc, int64_t tiles) {
+    supmask_block(F, A, Rc, tiles, blockIdx.x);
 }
 
 // one target's six features (tgt_lin rows)
@@ -175,11 +189,11 @@ struct alignas(16) LinFeatures {
 // group back to back, so the tiles' records are fetched into one die's L2
 // and reused there by every super-block.  Workgroups past the last tile
 // group (padding to a multiple of kXcds) return at once.
-__global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S, Records Rc, int64_t quads) {
+__device__ __forceinline__ void detect_block(const FanConst& F, const Arrays& A, const Slots& S, const Records& Rc,
+                                             int64_t quads, int64_t L) {
     __shared__ unsigned long long s_hit[kBlock / 64][kHitQueue];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const int64_t L = blockIdx.x;
     const int64_t k = L / kXcds;
     const int64_t quad = L % kXcds + kXcds * (k / F.nsup);
     const int64_t sup = k % F.nsup;
@@ -247,7 +261,7 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
         }
     }
     boxes_tested += (unsigned long long)__popcll(__ballot(active)) * (unsigned long long)nbk;
-    if (lane == 0 && boxes_tested) atomicAdd(&Rc.scan[(blockIdx.x % kScanSlots) * 2 + 1], boxes_tested);
+    if (lane == 0 && boxes_tested) atomicAdd(&Rc.scan[(L % kScanSlots) * 2 + 1], boxes_tested);
     if (!__any(active)) return;  // wave-uniform; no barriers below
     // the blocks each ray keeps, decided for all blocks before any pair test
     unsigned long long keep[kSuperBlock];
@@ -325,9 +339,13 @@ __global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S
     }
     if (qn) flush();
     if (lane == 0) {
-        if (tested) atomicAdd(&Rc.scan[(blockIdx.x % kScanSlots) * 2], tested);
+        if (tested) atomicAdd(&Rc.scan[(L % kScanSlots) * 2], tested);
         if (hits) atomicAdd(Rc.hit_count, hits);
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S, Records Rc, int64_t quads) {
+    detect_block(F, A, S, Rc, quads, blockIdx.x);
 }
 
 // eigenray_collection::sum_eigenrays, one thread per target
@@ -403,6 +421,116 @@ __global__ void __launch_bounds__(kBlock) k_build(const KParams* __restrict__ P,
     double* lds = group >= 4 ? s_cube + (threadIdx.x / group) * 54 : nullptr;
     for (int64_t c = gid / group; c < ncand; c += (int64_t)gridDim.x * blockDim.x / group)
         body_build<true>(c, lane_in_group, group, lds, P->E, P->F, P->A, S, Rc, time, step_key);
+}
+
+// ---- fused launches of a batch of fans (wq3d_batch_*) --------------------
+// One launch covers every fan of the batch: each fan owns a contiguous range
+// of the launch's blocks (first[g] .. first[g + 1] for the g-th fan of the
+// launch's list, fan id ids[g]), so a block belongs to one fan and runs that
+// fan's body with the fan's parameters (kps[fan], its device copy).  The
+// fans' times and step counts live in device memory (clk): k_bounce_multi
+// reads them before the step's rotation, k_advance_multi advances them with
+// the host's accumulation (time += dt), k_build_multi reads them after, as
+// wq3d_step passes them - so a captured graph of many steps replays with
+// fixed arguments.
+__device__ __forceinline__ int fan_of_block(const int64_t* __restrict__ first, int n, int64_t b) {
+    int lo = 0, hi = n;  // first[lo] <= b < first[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (first[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <int PK, bool FLAT>
+__global__ void __launch_bounds__(kBlock) k_bounce_multi(const KParams* const* __restrict__ kps,
+                                                        const int32_t* __restrict__ ids,
+                                                        const int64_t* __restrict__ first, int n, Slots S,
+                                                        const FanClock* __restrict__ clk) {
+    const int g = fan_of_block(first, n, blockIdx.x);
+    const int f = ids[g];
+    const KParams* P = kps[f];
+    const int64_t i = (int64_t)(blockIdx.x - first[g]) * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *P->R.cand_count = 0ull;
+        *P->R.hit_count = 0ull;
+        *P->R.pass_count = 0ull;
+    }
+    if (i >= P->F.N) return;
+    body_reflect_light(i, P->F, P->A, S);
+    if (P->A.flag[i] == 0 && !(P->F.want_verbs && P->E.nvol > 0)) return;
+    body_bounce<KindsOf<PK, FLAT>>(i, P->E, P->F, P->A, S, P->R, clk[f].time, clk[f].step);
+}
+
+template <int PK, bool FLAT>
+__global__ void __launch_bounds__(kBlock) k_advance_multi(const KParams* const* __restrict__ kps,
+                                                         const int32_t* __restrict__ ids,
+                                                         const int64_t* __restrict__ first, int n, Slots S,
+                                                         FanClock* __restrict__ clk) {
+    const int g = fan_of_block(first, n, blockIdx.x);
+    const int f = ids[g];
+    const KParams* P = kps[f];
+    const int64_t i = (int64_t)(blockIdx.x - first[g]) * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        clk[f].time += clk[f].dt;
+        clk[f].step += 1;
+    }
+    if (i < P->F.N) {
+        body_advance<KindsOf<PK, FLAT>>(i, P->E, P->F, P->A, S);
+        store_window(P->F, P->A, S, i);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_supmask_multi(const KParams* const* __restrict__ kps,
+                                                         const int32_t* __restrict__ ids,
+                                                         const int64_t* __restrict__ first, int n) {
+    const int g = fan_of_block(first, n, blockIdx.x);
+    const KParams* P = kps[ids[g]];
+    supmask_block(P->F, P->A, P->R, win_tiles(P->F), blockIdx.x - first[g]);
+}
+
+// each fan's range is a whole detect grid, a multiple of kXcds blocks, so a
+// block's die (its index mod kXcds) is the one the single-fan grid gives it
+__global__ void __launch_bounds__(kBlock) k_detect_multi(const KParams* const* __restrict__ kps,
+                                                        const int32_t* __restrict__ ids,
+                                                        const int64_t* __restrict__ first, int n, Slots S) {
+    const int g = fan_of_block(first, n, blockIdx.x);
+    const KParams* P = kps[ids[g]];
+    const int64_t tiles_az = (P->F.naz + kTile - 1) / kTile;
+    const int64_t tiles = ((P->F.nde + kTile - 1) / kTile) * tiles_az;
+    detect_block(P->F, P->A, S, P->R, (tiles + 3) / 4, blockIdx.x - first[g]);
+}
+
+// the candidates of every fan as one list: a block reads the fans' counts,
+// and each group of `group` lanes takes candidates of the whole list
+
+__global__ void __launch_bounds__(kBlock) k_build_multi(const KParams* const* __restrict__ kps,
+                                                       const int32_t* __restrict__ ids, int n, Slots S,
+                                                       const FanClock* __restrict__ clk, int group) {
+    __shared__ int64_t s_first[kMaxBatchFans + 1];
+    __shared__ double s_cube[(kBlock / 4) * 54];
+    for (int g = threadIdx.x; g < n; g += blockDim.x) {
+        const Records& R = kps[ids[g]]->R;
+        const unsigned long long raw = *R.cand_count;
+        s_first[g + 1] = (int64_t)(raw < (unsigned long long)R.cand_cap ? raw : (unsigned long long)R.cand_cap);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_first[0] = 0;
+        for (int g = 0; g < n; ++g) s_first[g + 1] += s_first[g];
+    }
+    __syncthreads();
+    const int64_t total = s_first[n];
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane_in_group = (int)(gid % group);
+    double* lds = s_cube + (threadIdx.x / group) * 54;
+    for (int64_t c = gid / group; c < total; c += (int64_t)gridDim.x * blockDim.x / group) {
+        const int g = fan_of_block(s_first, n, c);
+        const int f = ids[g];
+        const KParams* P = kps[f];
+        body_build<true>(c - s_first[g], lane_in_group, group, lds, P->E, P->F, P->A, S, P->R, clk[f].time,
+                         clk[f].step);
+    }
 }
 #endif  // WQ3D_EMULATE
 
@@ -1117,7 +1245,8 @@ void use_record_set(wq3d_fan* f, int s) {
 
 // copy record set `s` to the host vectors (its last writer has finished,
 // see `done`) and clear its counters on the stream, before any later launch
-wq3d_status drain_record_set(wq3d_fan* f, int s) {
+wq3d_status drain_record_set(wq3d_fan* f, int s, hipStream_t st = nullptr) {
+    if (!st) st = f->stream;
     wq3d_fan::RecSet& r = f->rs[s];
     HIP_TRY(rt::event_sync(r.done));
     unsigned long long cnt[3];
@@ -1149,7 +1278,7 @@ wq3d_status drain_record_set(wq3d_fan* f, int s) {
         f->h_refl.resize(n0 + cnt[2]);
         HIP_TRY(rt::memcpy(f->h_refl.data() + n0, r.refl, cnt[2] * sizeof(wq3d_reflection), hipMemcpyDeviceToHost));
     }
-    HIP_TRY(rt::memset_async(r.count, 0, 3 * sizeof(unsigned long long), f->stream));
+    HIP_TRY(rt::memset_async(r.count, 0, 3 * sizeof(unsigned long long), st));
     r.pending = false;
     return WQ3D_OK;
 }
@@ -1158,13 +1287,14 @@ wq3d_status drain_record_set(wq3d_fan* f, int s) {
 // the other set, and copy that one out first if it still holds the batch
 // before (finished long ago, so this host work overlaps the kernels queued
 // since)
-wq3d_status rotate_record_sets(wq3d_fan* f) {
+wq3d_status rotate_record_sets(wq3d_fan* f, hipStream_t stream = nullptr) {
+    if (!stream) stream = f->stream;
     wq3d_fan::RecSet& cur = f->rs[f->rset];
-    HIP_TRY(rt::event_record(cur.done, f->stream));
+    HIP_TRY(rt::event_record(cur.done, stream));
     cur.pending = true;
     const int next = f->rset ^ 1;
     if (f->rs[next].pending) {
-        wq3d_status st = drain_record_set(f, next);
+        wq3d_status st = drain_record_set(f, next, stream);
         if (st) return st;
     }
     use_record_set(f, next);
@@ -2283,3 +2413,5 @@ int32_t wq3d_emu_check_detect(int64_t num_rays, int64_t num_targets, uint64_t se
 #endif
 
 }  // extern "C"
+
+#include "wq3d_batch.h"

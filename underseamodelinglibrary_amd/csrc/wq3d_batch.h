@@ -1,0 +1,498 @@
+// wq3d_batch.h - many fans stepped together (part of wq3d.hip's translation
+// unit; included at its end).
+//
+// The reference runs independent fans as wavefront_generator tasks on a CPU
+// thread pool (wavegen/wavefront_generator.cc:52-127, threads/
+// thread_pool.cc:21-75), each task stepping its own wave_queue.  A batch
+// steps every fan of a list with one launch per kernel of the step instead
+// of one per fan: the fused launches (k_*_multi) give each fan a contiguous
+// range of blocks, so small fans fill the device together.  Every 64 steps
+// the record sets of all fans rotate, exactly as each fan's own wq3d_step
+// would rotate them; between rotations the 64 steps' launches are captured
+// once per record set in a HIP graph and replayed.  Launch shapes (threads
+// per block of the per-ray launches, blocks of the construction launch) are
+// picked by wq3d_batch_tune, which times real steps of the batch: the shape
+// changes only how work is spread over the device, never a result.
+#include <hip/hip_runtime.h>
+
+namespace {
+
+constexpr int kBatchGraphSteps = 64;  // the record sets' rotation period
+
+// a device copy of n elements that stays writable (DeviceBuffers::upload
+// hands out const pointers)
+template <class T>
+hipError_t put(DeviceBuffers& mem, T** out, const T* src, size_t n) {
+    hipError_t e = mem.alloc(out, n);
+    if (e == hipSuccess && n) e = rt::memcpy(*out, src, n * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+
+bool identity_slots(const wq3d_fan* f) {
+    return f->phys[0] == 0 && f->phys[1] == 1 && f->phys[2] == 2 && f->phys[3] == 3;
+}
+
+}  // namespace
+
+struct wq3d_batch {
+    std::vector<wq3d_fan*> fans;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DeviceBuffers mem;
+    const KParams** kps = nullptr;  // device [2][n]: fan f's parameters for record set r at r * n + f
+    FanClock* clk = nullptr;        // device [n]
+    struct Group {                  // fans of one environment variant (per-ray launches)
+        int variant = 0;
+        std::vector<int32_t> ids;
+        int32_t* d_ids = nullptr;
+        int64_t* d_first = nullptr;
+        int64_t blocks = 0;
+    };
+    std::vector<Group> groups;
+    std::vector<int32_t> tids;  // fans with targets (the eigenray search)
+    int32_t* d_tids = nullptr;
+    int64_t *d_sup_first = nullptr, *d_det_first = nullptr;
+    int64_t sup_blocks = 0, det_blocks = 0;
+    wq3d_batch_shape shape{kBlock, 1024, 1, 0};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+void batch_drop_graphs(wq3d_batch* b) {
+#ifndef WQ3D_EMULATE
+    for (auto& e : b->exec) {
+        if (e) (void)hipGraphExecDestroy(e);
+        e = nullptr;
+    }
+#endif
+}
+
+// block ranges of the per-ray launches for the current threads per block
+wq3d_status batch_tables(wq3d_batch* b) {
+    const int64_t tpb = b->shape.ray_threads;
+    for (auto& g : b->groups) {
+        std::vector<int64_t> first(g.ids.size() + 1, 0);
+        for (size_t k = 0; k < g.ids.size(); ++k)
+            first[k + 1] = first[k] + (b->fans[g.ids[k]]->F.N + tpb - 1) / tpb;
+        g.blocks = first.back();
+        HIP_TRY(rt::memcpy(g.d_first, first.data(), first.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    }
+    return WQ3D_OK;
+}
+
+// one step's launches on the batch's stream for record set `rset`; the
+// host state (slots, times, record sets) is advanced by the caller
+wq3d_status batch_launch_step(wq3d_batch* b, const Slots& S, int rset) {
+    const int n = (int)b->fans.size();
+    const KParams* const* kps = b->kps + (size_t)rset * n;
+#ifndef WQ3D_EMULATE
+    const unsigned tpb = (unsigned)b->shape.ray_threads;
+    for (const auto& g : b->groups) {
+        const int ng = (int)g.ids.size();
+        switch (g.variant) {
+        case 1:
+            hipLaunchKernelGGL((k_bounce_multi<WQ3D_PROFILE_MUNK, true>), dim3((unsigned)g.blocks), dim3(tpb), 0,
+                               b->stream, kps, g.d_ids, g.d_first, ng, S, b->clk);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_bounce_multi<WQ3D_PROFILE_LINEAR, true>), dim3((unsigned)g.blocks), dim3(tpb), 0,
+                               b->stream, kps, g.d_ids, g.d_first, ng, S, b->clk);
+            break;
+        case 3:
+            hipLaunchKernelGGL((k_bounce_multi<kProfileGrid1Linear, true>), dim3((unsigned)g.blocks), dim3(tpb), 0,
+                               b->stream, kps, g.d_ids, g.d_first, ng, S, b->clk);
+            break;
+        default:
+            return fail(WQ3D_ERR_UNSUPPORTED, "environment kind without a batch variant");
+        }
+    }
+    // the rotation (past <- prev <- curr <- next <- past)
+    const Slots R{S.prev, S.curr, S.next, S.past};
+    for (const auto& g : b->groups) {
+        const int ng = (int)g.ids.size();
+        switch (g.variant) {
+        case 1:
+            hipLaunchKernelGGL((k_advance_multi<WQ3D_PROFILE_MUNK, true>), dim3((unsigned)g.blocks), dim3(tpb), 0,
+                               b->stream, kps, g.d_ids, g.d_first, ng, R, b->clk);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_advance_multi<WQ3D_PROFILE_LINEAR, true>), dim3((unsigned)g.blocks), dim3(tpb), 0,
+                               b->stream, kps, g.d_ids, g.d_first, ng, R, b->clk);
+            break;
+        default:
+            hipLaunchKernelGGL((k_advance_multi<kProfileGrid1Linear, true>), dim3((unsigned)g.blocks), dim3(tpb),
+                               0, b->stream, kps, g.d_ids, g.d_first, ng, R, b->clk);
+            break;
+        }
+    }
+    if (!b->tids.empty()) {
+        const int nt = (int)b->tids.size();
+        hipLaunchKernelGGL(k_supmask_multi, dim3((unsigned)b->sup_blocks), dim3(kBlock), 0, b->stream, kps,
+                           b->d_tids, b->d_sup_first, nt);
+        hipLaunchKernelGGL(k_detect_multi, dim3((unsigned)b->det_blocks), dim3(kBlock), 0, b->stream, kps,
+                           b->d_tids, b->d_det_first, nt, R);
+        hipLaunchKernelGGL(k_build_multi, dim3((unsigned)b->shape.build_blocks), dim3(kBlock), 0, b->stream, kps,
+                           b->d_tids, nt, R, b->clk, kBuildLanes);
+    }
+    HIP_TRY(rt::last_error());
+#else
+    // the emulation build steps the fans one after another through the
+    // single-fan launch paths, with the same slots, times and record sets
+    (void)kps;
+    const Slots R{S.prev, S.curr, S.next, S.past};
+    for (wq3d_fan* f : b->fans) {
+        const unsigned grid = (unsigned)((f->F.N + kBlock - 1) / kBlock);
+        launch_bounce(b->stream, grid, f->kp + rset, f->env->dev, f->F, f->A, S, f->R, f->time, f->steps);
+    }
+    for (wq3d_fan* f : b->fans) {
+        const unsigned grid = (unsigned)((f->F.N + kBlock - 1) / kBlock);
+        launch_advance(b->stream, grid, f->env->dev, f->F, f->A, R);
+        if (f->F.nt > 0 && f->F.has_spreading) {
+            const int64_t tiles = ((f->F.nde + kTile - 1) / kTile) * ((f->F.naz + kTile - 1) / kTile);
+            launch_detect(b->stream, (tiles + 3) / 4, f->F, f->A, R, f->R);
+            double t = f->time + f->F.dt;
+            launch_build(b->stream, f->kp + rset, f->env->dev, f->F, f->A, R, f->R, t, f->steps + 1);
+        }
+    }
+#endif
+    return WQ3D_OK;
+}
+
+// the host side of one step for every fan: the rotation, time and step
+// count as wq3d_step keeps them, and the record sets every 64 steps
+wq3d_status batch_host_step(wq3d_batch* b) {
+    for (wq3d_fan* f : b->fans) {
+        const int save = f->phys[0];
+        f->phys[0] = f->phys[1];
+        f->phys[1] = f->phys[2];
+        f->phys[2] = f->phys[3];
+        f->phys[3] = save;
+        f->time += f->F.dt;
+        f->steps += 1;
+        f->pending_steps += 1;
+    }
+    if (b->fans[0]->pending_steps >= kBatchGraphSteps) {
+        for (wq3d_fan* f : b->fans) {
+            wq3d_status st = rotate_record_sets(f, b->stream);
+            if (st) return st;
+        }
+    }
+    return WQ3D_OK;
+}
+
+wq3d_status batch_eager_step(wq3d_batch* b) {
+    wq3d_fan* f0 = b->fans[0];
+    wq3d_status st = batch_launch_step(b, slots_of(f0), f0->rset);
+    if (st) return st;
+#ifdef WQ3D_EMULATE
+    // the emulated launches read the host clocks directly
+#endif
+    return batch_host_step(b);
+}
+
+#ifndef WQ3D_EMULATE
+// 64 steps from the identity slots for record set `rset`, captured once
+wq3d_status batch_graph(wq3d_batch* b, int rset, hipGraphExec_t* out) {
+    if (b->exec[rset]) {
+        *out = b->exec[rset];
+        return WQ3D_OK;
+    }
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int phys[4] = {0, 1, 2, 3};
+    wq3d_status st = WQ3D_OK;
+    for (int k = 0; k < kBatchGraphSteps && !st; ++k) {
+        st = batch_launch_step(b, Slots{phys[0], phys[1], phys[2], phys[3]}, rset);
+        const int save = phys[0];
+        phys[0] = phys[1];
+        phys[1] = phys[2];
+        phys[2] = phys[3];
+        phys[3] = save;
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (st) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return fail(WQ3D_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&b->exec[rset], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(WQ3D_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    *out = b->exec[rset];
+    return WQ3D_OK;
+}
+#endif
+
+wq3d_status batch_check_aligned(const wq3d_batch* b) {
+    const wq3d_fan* f0 = b->fans[0];
+    for (const wq3d_fan* f : b->fans) {
+        if (f->steps != f0->steps || f->pending_steps != f0->pending_steps || f->rset != f0->rset ||
+            f->phys[0] != f0->phys[0] || f->phys[1] != f0->phys[1] || f->phys[2] != f0->phys[2])
+            return fail(WQ3D_ERR_STATE, "the batch's fans are no longer at the same step (stepped alone?)");
+    }
+    return WQ3D_OK;
+}
+
+// the fans' parameter copies and clocks as the host holds them, before a
+// batch call's first launch (the stream is idle between calls)
+wq3d_status batch_prepare(wq3d_batch* b) {
+    wq3d_status st = batch_check_aligned(b);
+    if (st) return st;
+    const size_t n = b->fans.size();
+    std::vector<FanClock> clk(n);
+    for (size_t k = 0; k < n; ++k) {
+        wq3d_fan* f = b->fans[k];
+        if (f->kp_dirty) {
+            HIP_TRY(rt::sync(f->stream));
+            const int active = f->rset;
+            for (int r = 0; r < 2; ++r) {
+                use_record_set(f, r);
+                const KParams kp{f->env->dev, f->F, f->A, f->R};
+                HIP_TRY(rt::memcpy(f->kp + r, &kp, sizeof(KParams), hipMemcpyHostToDevice));
+            }
+            use_record_set(f, active);
+            f->kp_dirty = false;
+        }
+        clk[k] = FanClock{f->time, f->steps, f->F.dt, 0};
+    }
+    HIP_TRY(rt::memcpy(b->clk, clk.data(), n * sizeof(FanClock), hipMemcpyHostToDevice));
+    return WQ3D_OK;
+}
+
+wq3d_status batch_run(wq3d_batch* b, int64_t n) {
+    int64_t done = 0;
+    while (done < n) {
+        wq3d_fan* f0 = b->fans[0];
+#ifndef WQ3D_EMULATE
+        if (b->shape.use_graph && n - done >= kBatchGraphSteps && f0->pending_steps == 0 && identity_slots(f0)) {
+            hipGraphExec_t exec = nullptr;
+            wq3d_status st = batch_graph(b, f0->rset, &exec);
+            if (st) return st;
+            HIP_TRY(hipGraphLaunch(exec, b->stream));
+            for (int k = 0; k < kBatchGraphSteps; ++k) {
+                st = batch_host_step(b);
+                if (st) return st;
+            }
+            done += kBatchGraphSteps;
+            continue;
+        }
+#endif
+        wq3d_status st = batch_eager_step(b);
+        if (st) return st;
+        done += 1;
+    }
+    return WQ3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+wq3d_status wq3d_batch_create(wq3d_fan* const* fans, int32_t num_fans, wq3d_batch** out) {
+    if (!fans || !out || num_fans < 1) return fail(WQ3D_ERR_ARGUMENT, "no fans");
+    *out = nullptr;
+    if (num_fans > kMaxBatchFans) return fail(WQ3D_ERR_UNSUPPORTED, "more than 1024 fans in a batch");
+    auto* b = new wq3d_batch();
+    for (int32_t k = 0; k < num_fans; ++k) {
+        wq3d_fan* f = fans[k];
+        if (!f) {
+            delete b;
+            return fail(WQ3D_ERR_ARGUMENT, "null fan");
+        }
+        for (wq3d_fan* g : b->fans)
+            if (g == f) {
+                delete b;
+                return fail(WQ3D_ERR_ARGUMENT, "a fan appears twice in the batch");
+            }
+        if (f->device != fans[0]->device) {
+            delete b;
+            return fail(WQ3D_ERR_ARGUMENT, "the batch's fans are on different devices");
+        }
+        if (f->timing) {
+            delete b;
+            return fail(WQ3D_ERR_UNSUPPORTED, "per-kernel timing is not available in a batch");
+        }
+        b->fans.push_back(f);
+    }
+    wq3d_status st = batch_check_aligned(b);
+    if (st) {
+        delete b;
+        return st;
+    }
+    b->device = fans[0]->device;
+    auto bail = [&](wq3d_status s) {
+        if (b->stream) (void)rt::stream_destroy(b->stream);
+        delete b;
+        return s;
+    };
+    if (rt::set_device(b->device) != hipSuccess || rt::stream_create(&b->stream) != hipSuccess)
+        return bail(fail(WQ3D_ERR_DEVICE, "batch stream creation failed"));
+    const int n = num_fans;
+    std::vector<const KParams*> kps((size_t)2 * n);
+    for (int k = 0; k < n; ++k) {
+        kps[k] = b->fans[k]->kp + 0;
+        kps[(size_t)n + k] = b->fans[k]->kp + 1;
+    }
+    if (put(b->mem, &b->kps, kps.data(), kps.size()) != hipSuccess || b->mem.alloc(&b->clk, (size_t)n) != hipSuccess)
+        return bail(fail(WQ3D_ERR_DEVICE, "batch table upload failed"));
+    // per-ray launches: fans grouped by environment variant
+    for (int k = 0; k < n; ++k) {
+        int v = 1;
+#ifndef WQ3D_EMULATE
+        v = env_variant(b->fans[k]->env->dev);
+        if (v == 0) return bail(fail(WQ3D_ERR_UNSUPPORTED, "environment kind without a batch variant"));
+#endif
+        auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const wq3d_batch::Group& g) { return g.variant == v; });
+        if (it == b->groups.end()) {
+            b->groups.push_back(wq3d_batch::Group{});
+            b->groups.back().variant = v;
+            it = b->groups.end() - 1;
+        }
+        it->ids.push_back(k);
+    }
+    for (auto& g : b->groups) {
+        if (put(b->mem, &g.d_ids, g.ids.data(), g.ids.size()) != hipSuccess ||
+            b->mem.alloc(&g.d_first, g.ids.size() + 1) != hipSuccess)
+            return bail(fail(WQ3D_ERR_DEVICE, "batch table upload failed"));
+    }
+    // the eigenray search: the fans with targets, each a whole grid of its own
+    std::vector<int64_t> sup{0}, det{0};
+    for (int k = 0; k < n; ++k) {
+        const FanConst& F = b->fans[k]->F;
+        if (!(F.nt > 0 && F.has_spreading)) continue;
+        b->tids.push_back(k);
+        const int64_t tiles = ((F.nde + kTile - 1) / kTile) * ((F.naz + kTile - 1) / kTile);
+        const int64_t quads = (tiles + 3) / 4;
+        sup.push_back(sup.back() + (tiles + kBlock / 64 - 1) / (kBlock / 64));
+        det.push_back(det.back() + kXcds * ((quads + kXcds - 1) / kXcds) * F.nsup);
+    }
+    b->sup_blocks = sup.back();
+    b->det_blocks = det.back();
+    if (!b->tids.empty() &&
+        (put(b->mem, &b->d_tids, b->tids.data(), b->tids.size()) != hipSuccess ||
+         put(b->mem, &b->d_sup_first, sup.data(), sup.size()) != hipSuccess ||
+         put(b->mem, &b->d_det_first, det.data(), det.size()) != hipSuccess))
+        return bail(fail(WQ3D_ERR_DEVICE, "batch table upload failed"));
+    st = batch_tables(b);
+    if (st) return bail(st);
+    for (auto& e : b->ev)
+        if (rt::event_create(&e) != hipSuccess) return bail(fail(WQ3D_ERR_DEVICE, "event creation failed"));
+    *out = b;
+    return WQ3D_OK;
+}
+
+wq3d_status wq3d_batch_destroy(wq3d_batch* b) {
+    if (!b) return WQ3D_OK;
+    (void)rt::set_device(b->device);
+    (void)rt::sync(b->stream);
+    batch_drop_graphs(b);
+    for (auto& e : b->ev)
+        if (e) (void)rt::event_destroy(e);
+    (void)rt::stream_destroy(b->stream);
+    delete b;
+    return WQ3D_OK;
+}
+
+wq3d_status wq3d_batch_step(wq3d_batch* b, int64_t n) {
+    if (!b) return fail(WQ3D_ERR_ARGUMENT, "null batch");
+    if (n < 0) return fail(WQ3D_ERR_ARGUMENT, "negative step count");
+    HIP_TRY(rt::set_device(b->device));
+    wq3d_status st = batch_prepare(b);
+    if (st) return st;
+    st = batch_run(b, n);
+    if (st) return st;
+    HIP_TRY(rt::sync(b->stream));
+    return WQ3D_OK;
+}
+
+wq3d_status wq3d_batch_get_shape(const wq3d_batch* b, wq3d_batch_shape* shape) {
+    if (!b || !shape) return fail(WQ3D_ERR_ARGUMENT, "null argument");
+    *shape = b->shape;
+    return WQ3D_OK;
+}
+
+wq3d_status wq3d_batch_set_shape(wq3d_batch* b, const wq3d_batch_shape* shape) {
+    if (!b || !shape) return fail(WQ3D_ERR_ARGUMENT, "null argument");
+    const int t = shape->ray_threads;
+    if (!(t == 64 || t == 128 || t == 256)) return fail(WQ3D_ERR_ARGUMENT, "ray_threads must be 64, 128 or 256");
+    if (shape->build_blocks < 1 || shape->build_blocks > (1 << 16))
+        return fail(WQ3D_ERR_ARGUMENT, "build_blocks must be in 1 .. 65536");
+    HIP_TRY(rt::set_device(b->device));
+    HIP_TRY(rt::sync(b->stream));
+    batch_drop_graphs(b);
+    b->shape = *shape;
+    b->shape.use_graph = shape->use_graph ? 1 : 0;
+    b->shape.pad_ = 0;
+    return batch_tables(b);
+}
+
+// The autotuner: the time per step of each launch shape over `trial` real
+// steps of the batch (eager launches, HIP events on the batch's stream),
+// first over threads per block of the per-ray launches, then over blocks of
+// the construction launch; the fastest shape is kept.  Each trial advances
+// the batch by `trial` steps - the results of every step are the same
+// whatever the shape.
+wq3d_status wq3d_batch_tune(wq3d_batch* b, int32_t trial, double* best_ms_per_step, int64_t* steps_taken) {
+    if (!b || trial < 1) return fail(WQ3D_ERR_ARGUMENT, "null batch or trial length below 1");
+    int64_t taken = 0;
+    if (steps_taken) *steps_taken = 0;
+    HIP_TRY(rt::set_device(b->device));
+    wq3d_status st = batch_prepare(b);
+    if (st) return st;
+    const int use_graph = b->shape.use_graph;
+    auto time_shape = [&](int threads, int blocks, double& ms) -> wq3d_status {
+        wq3d_batch_shape s{threads, blocks, 0, 0};
+        wq3d_status e = wq3d_batch_set_shape(b, &s);
+        if (e) return e;
+        // one untimed step settles the new shape
+        e = batch_eager_step(b);
+        if (e) return e;
+        taken += 1;
+        if (steps_taken) *steps_taken = taken;
+        HIP_TRY(rt::event_record(b->ev[0], b->stream));
+        for (int k = 0; k < trial; ++k) {
+            e = batch_eager_step(b);
+            if (e) return e;
+            taken += 1;
+            if (steps_taken) *steps_taken = taken;
+        }
+        HIP_TRY(rt::event_record(b->ev[1], b->stream));
+        HIP_TRY(rt::event_sync(b->ev[1]));
+        float t = 0.0f;
+        HIP_TRY(rt::event_elapsed(&t, b->ev[0], b->ev[1]));
+        ms = (double)t / trial;
+        return WQ3D_OK;
+    };
+    int best_threads = b->shape.ray_threads, best_blocks = b->shape.build_blocks;
+    double best = 1e300;
+    for (int threads : {64, 128, 256}) {
+        double ms = 0.0;
+        st = time_shape(threads, best_blocks, ms);
+        if (st) return st;
+        if (ms < best) {
+            best = ms;
+            best_threads = threads;
+        }
+    }
+    if (!b->tids.empty()) {
+        for (int blocks : {256, 512, 1024, 2048, 4096}) {
+            double ms = 0.0;
+            st = time_shape(best_threads, blocks, ms);
+            if (st) return st;
+            if (ms < best) {
+                best = ms;
+                best_blocks = blocks;
+            }
+        }
+    }
+    wq3d_batch_shape s{best_threads, best_blocks, use_graph, 0};
+    st = wq3d_batch_set_shape(b, &s);
+    if (st) return st;
+    HIP_TRY(rt::sync(b->stream));
+    if (best_ms_per_step) *best_ms_per_step = best;
+    return WQ3D_OK;
+}
+
+}  // extern "C"

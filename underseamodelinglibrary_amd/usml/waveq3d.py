@@ -564,3 +564,57 @@ class wave_queue(queue_base):
 
     def sync(self):
         self._lib.check(self._lib.wq3d_sync(self._fan))
+
+
+class fan_batch:
+    """Many device wave_queues stepped together through wq3d_batch_*: one
+    fused launch per kernel of a step for every queue, 64-step loops replayed
+    as HIP graphs.  step(n) is n calls of step() on every queue, with each
+    queue's listener calls replayed in its own order afterwards, as
+    wave_queue.step(n) replays them.  The queues must be fresh (or at the
+    same step) and are stepped only through the batch while it is open."""
+
+    def __init__(self, queues, lib=None):
+        self._queues = list(queues)
+        if not self._queues:
+            raise ValueError("a batch needs at least one wave_queue")
+        self._lib = lib if lib is not None else self._queues[0]._lib
+        fans = (C.c_void_p * len(self._queues))(*[q._fan.value for q in self._queues])
+        self._h = C.c_void_p()
+        self._lib.check(self._lib.wq3d_batch_create(fans, len(self._queues), C.byref(self._h)))
+
+    def step(self, n=1):
+        self._lib.check(self._lib.wq3d_batch_step(self._h, int(n)))
+        for q in self._queues:
+            q.step_end(int(n))
+
+    def shape(self):
+        s = _abi.BatchShape()
+        self._lib.check(self._lib.wq3d_batch_get_shape(self._h, C.byref(s)))
+        return {"ray_threads": s.ray_threads, "build_blocks": s.build_blocks, "use_graph": bool(s.use_graph)}
+
+    def set_shape(self, ray_threads=256, build_blocks=1024, use_graph=True):
+        s = _abi.BatchShape(ray_threads, build_blocks, 1 if use_graph else 0, 0)
+        self._lib.check(self._lib.wq3d_batch_set_shape(self._h, C.byref(s)))
+
+    def tune(self, trial=8):
+        """Run the autotuner over `trial` real steps per candidate shape (the
+        queues advance by those steps, their listener calls replayed);
+        returns the chosen shape and its time per step in ms."""
+        ms = C.c_double()
+        taken = C.c_int64()
+        self._lib.check(self._lib.wq3d_batch_tune(self._h, int(trial), C.byref(ms), C.byref(taken)))
+        for q in self._queues:
+            q.step_end(taken.value)
+        return self.shape(), ms.value
+
+    def close(self):
+        if self._h.value:
+            self._lib.wq3d_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
