@@ -276,6 +276,13 @@ wq3d_status wq3d_batch_destroy(wq3d_batch* batch);
 wq3d_status wq3d_batch_step(wq3d_batch* batch, int64_t n);
 wq3d_status wq3d_batch_get_shape(const wq3d_batch* batch, wq3d_batch_shape* shape);
 wq3d_status wq3d_batch_set_shape(wq3d_batch* batch, const wq3d_batch_shape* shape);
+/* HIP events around each step's search stage (the fused launches of the
+ * tiles' super-block masks and of the eigenray search filter) on the
+ * batch's stream; while on, steps are launched one by one (no graph).
+ * wq3d_batch_stage_times returns the mean stage time per step since the
+ * last call and the number of steps it covers, then resets them. */
+wq3d_status wq3d_batch_set_timing(wq3d_batch* batch, int32_t enable);
+wq3d_status wq3d_batch_stage_times(wq3d_batch* batch, double* ms_per_step, int64_t* steps);
 /* The autotuner: times `trial` real steps of the batch for each candidate
  * shape (threads per block, then construction blocks), keeps the fastest and
  * returns its time per step.  Advances the batch by the steps it times plus

@@ -66,6 +66,8 @@ def bind(lib):
         "wq3d_batch_get_shape": ([vp, C.POINTER(_abi.BatchShape)], i32),
         "wq3d_batch_set_shape": ([vp, C.POINTER(_abi.BatchShape)], i32),
         "wq3d_batch_tune": ([vp, i32, dp, C.POINTER(i64)], i32),
+        "wq3d_batch_set_timing": ([vp, i32], i32),
+        "wq3d_batch_stage_times": ([vp, dp, C.POINTER(i64)], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)

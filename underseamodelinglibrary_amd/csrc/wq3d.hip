@@ -2190,6 +2190,9 @@ wq3d_status wq3d_search_totals(wq3d_fan* f, int64_t* pairs_tested, int64_t* box_
         *box_tests += (int64_t)c[2 * k + 1];
     }
     HIP_TRY(rt::memset_async(f->R.scan, 0, sizeof(c), f->stream));
+    // cleared before the call returns: a batch's launches on another stream
+    // may count into these next
+    HIP_TRY(rt::sync(f->stream));
     return WQ3D_OK;
 }
 

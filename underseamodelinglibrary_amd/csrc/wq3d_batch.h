@@ -56,6 +56,11 @@ struct wq3d_batch {
     wq3d_batch_shape shape{kBlock, 1024, 1, 0};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // timing of the search stage (fused supmask + detect launches)
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> stage_events, event_pool;
+    double stage_ms = 0.0;
+    int64_t stage_steps = 0;
 };
 
 namespace {
@@ -129,10 +134,25 @@ wq3d_status batch_launch_step(wq3d_batch* b, const Slots& S, int rset) {
     }
     if (!b->tids.empty()) {
         const int nt = (int)b->tids.size();
+        std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+        if (b->timing) {
+            if (!b->event_pool.empty()) {
+                ev = b->event_pool.back();
+                b->event_pool.pop_back();
+            } else {
+                HIP_TRY(rt::event_create(&ev.first));
+                HIP_TRY(rt::event_create(&ev.second));
+            }
+            HIP_TRY(rt::event_record(ev.first, b->stream));
+        }
         hipLaunchKernelGGL(k_supmask_multi, dim3((unsigned)b->sup_blocks), dim3(kBlock), 0, b->stream, kps,
                            b->d_tids, b->d_sup_first, nt);
         hipLaunchKernelGGL(k_detect_multi, dim3((unsigned)b->det_blocks), dim3(kBlock), 0, b->stream, kps,
                            b->d_tids, b->d_det_first, nt, R);
+        if (b->timing) {
+            HIP_TRY(rt::event_record(ev.second, b->stream));
+            b->stage_events.push_back(ev);
+        }
         hipLaunchKernelGGL(k_build_multi, dim3((unsigned)b->shape.build_blocks), dim3(kBlock), 0, b->stream, kps,
                            b->d_tids, nt, R, b->clk, kBuildLanes);
     }
@@ -266,7 +286,7 @@ wq3d_status batch_run(wq3d_batch* b, int64_t n) {
     while (done < n) {
         wq3d_fan* f0 = b->fans[0];
 #ifndef WQ3D_EMULATE
-        if (b->shape.use_graph && n - done >= kBatchGraphSteps && f0->pending_steps == 0 && identity_slots(f0)) {
+        if (b->shape.use_graph && !b->timing && n - done >= kBatchGraphSteps && f0->pending_steps == 0 && identity_slots(f0)) {
             hipGraphExec_t exec = nullptr;
             wq3d_status st = batch_graph(b, f0->rset, &exec);
             if (st) return st;
@@ -388,6 +408,11 @@ wq3d_status wq3d_batch_destroy(wq3d_batch* b) {
     (void)rt::set_device(b->device);
     (void)rt::sync(b->stream);
     batch_drop_graphs(b);
+    for (auto* v : {&b->stage_events, &b->event_pool})
+        for (auto& e : *v) {
+            (void)rt::event_destroy(e.first);
+            (void)rt::event_destroy(e.second);
+        }
     for (auto& e : b->ev)
         if (e) (void)rt::event_destroy(e);
     (void)rt::stream_destroy(b->stream);
@@ -404,6 +429,31 @@ wq3d_status wq3d_batch_step(wq3d_batch* b, int64_t n) {
     st = batch_run(b, n);
     if (st) return st;
     HIP_TRY(rt::sync(b->stream));
+    return WQ3D_OK;
+}
+
+wq3d_status wq3d_batch_set_timing(wq3d_batch* b, int32_t enable) {
+    if (!b) return fail(WQ3D_ERR_ARGUMENT, "null batch");
+    b->timing = enable != 0;
+    return WQ3D_OK;
+}
+
+wq3d_status wq3d_batch_stage_times(wq3d_batch* b, double* ms_per_step, int64_t* steps) {
+    if (!b || !ms_per_step || !steps) return fail(WQ3D_ERR_ARGUMENT, "null argument");
+    HIP_TRY(rt::set_device(b->device));
+    HIP_TRY(rt::sync(b->stream));
+    for (auto& e : b->stage_events) {
+        float t = 0.0f;
+        HIP_TRY(rt::event_elapsed(&t, e.first, e.second));
+        b->stage_ms += t;
+        b->stage_steps += 1;
+        b->event_pool.push_back(e);
+    }
+    b->stage_events.clear();
+    *steps = b->stage_steps;
+    *ms_per_step = b->stage_steps ? b->stage_ms / (double)b->stage_steps : 0.0;
+    b->stage_ms = 0.0;
+    b->stage_steps = 0;
     return WQ3D_OK;
 }
 
