@@ -178,7 +178,7 @@ def tune(lib, n, trial=8):
 
 
 def leg_config1(lib):
-    """Config 1 at full size (BASELINE's CPU-runnable case): one fan, plain
+    """Config 1 at full size (the CPU-runnable case): one fan, plain
     per-step launches, the whole 600-step run timed after fan creation."""
     from underseamodelinglibrary_amd import scenarios
     from underseamodelinglibrary_amd.usml.waveq3d import wave_queue
