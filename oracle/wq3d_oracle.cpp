@@ -556,12 +556,16 @@ struct Spreading {
     Queue* w = nullptr;
     std::vector<double> norm_de, norm_az, init_area, spread, I_de, I_az, bw;
     std::vector<uint8_t> dup;
+    bool classic = false;     // spreading_ray (CLASSIC_RAY) instead of the hybrid Gaussian model
+    double init_c = 0.0;      // spreading_ray::_init_sound_speed
     static constexpr double OVERLAP = 2.0;
     static constexpr double THRESHOLD = 1.002305238;
     void init(Queue* q);
     void gaussian(double d, double wdt, double A, double* out);
     void intensity(double trho, double ttheta, double tphi, size_t de, size_t az,
                    const double offset[3], const double distance[3], double* result);
+    void intensity_classic(double trho, double ttheta, double tphi, size_t de, size_t az,
+                           const double offset[3], double* result);
     void intensity_de(size_t de, size_t az, const double offset[3], const double distance[3]);
     void intensity_az(size_t de, size_t az, const double offset[3], const double distance[3]);
     double width_de(size_t de, size_t az, const double offset[3]) const;
@@ -739,9 +743,13 @@ struct Queue {
         if (nt1 * nt2 > 0 && !has_spreading && naz() >= 2 && az_boundary && nde() >= 3)
             throw Error{"targets need a fan of at least 3 x 3 rays"};
         if (has_spreading) {
-            if (fd.spreading != WQ3D_SPREADING_HYBRID_GAUSSIAN)
-                throw Error{"only HYBRID_GAUSSIAN spreading is supported"};
+            if (fd.spreading != WQ3D_SPREADING_HYBRID_GAUSSIAN && fd.spreading != WQ3D_SPREADING_CLASSIC_RAY)
+                throw Error{"unknown spreading model"};
             spreading.init(this);
+            spreading.classic = fd.spreading == WQ3D_SPREADING_CLASSIC_RAY;
+            // spreading_ray.cc:41: the curr front's sound speed of ray (0, 0)
+            // once the queue is built
+            spreading.init_c = curr->c[curr->id(0, 0)];
         }
     }
 
@@ -1559,6 +1567,7 @@ void Spreading::init(Queue* q) {
             norm_az[d * naz + a] = init_area[d * naz + a] / norm_de[d];
         }
         norm_az[d * naz + naz - 1] = norm_az[d * naz + naz - 2];
+        init_area[d * naz + naz - 1] = init_area[d * naz + naz - 2];  // spreading_ray.cc:35
     }
     norm_de[nde - 1] = norm_de[nde - 2];
     for (size_t a = 0; a < naz - 1; ++a) {
@@ -1577,9 +1586,55 @@ void Spreading::gaussian(double d, double wdt, double A, double* out) {
     }
 }
 
+static double ray_distance(const Front& w, size_t i, size_t j);
+
+// wvector1::area of four points (types/wvector1.cc:151-169): triangles
+// (p1, p2, p3) and (p1, p3, p4) by Heron's formula
+static double quad_area(const Front& f, size_t i1, size_t i2, size_t i3, size_t i4) {
+    double a = ray_distance(f, i1, i2);
+    const double b = ray_distance(f, i1, i3);
+    double c = ray_distance(f, i2, i3);
+    const double area1 = 0.25 * std::sqrt((a + b + c) * (b + c - a) * (c + a - b) * (a + b - c));
+    a = ray_distance(f, i1, i4);
+    c = ray_distance(f, i4, i3);
+    const double area2 = 0.25 * std::sqrt((a + b + c) * (b + c - a) * (c + a - b) * (a + b - c));
+    return area1 + area2;
+}
+
+// spreading_ray::intensity (waveq3d/spreading_ray.cc:47-130): the ratio of
+// the ray tube's initial area to its area now (two triangles per cell,
+// interpolated in time toward prev or next), times c(target) / c(source);
+// zero more than four beam widths outside the fan.  The reference steps to
+// the lower cell with a size_t `--az`, which at az = 0 of a fan closing on
+// itself indexes outside the front; here that cell is the fan's wrap,
+// naz - 2, as the hybrid model's intensity takes it.
+void Spreading::intensity_classic(double trho, double ttheta, double tphi, size_t de, size_t az,
+                                  const double offset[3], double* result) {
+    if (offset[1] < 0.0) --de;
+    if (offset[2] < 0.0) az = (az > 0) ? az - 1 : w->naz() - 2;
+    double c;
+    w->profile.sound_speed(trho, ttheta, tphi, w->R, &c, nullptr, nullptr, nullptr);
+    const Front& p1 = *w->curr;
+    const double area1 = quad_area(p1, p1.id(de, az), p1.id(de + 1, az), p1.id(de + 1, az + 1), p1.id(de, az + 1));
+    const Front& p2 = (offset[0] < 0.0) ? *w->prev : *w->next;
+    const double area2 = quad_area(p2, p2.id(de, az), p2.id(de + 1, az), p2.id(de + 1, az + 1), p2.id(de, az + 1));
+    double u = std::fabs(offset[0]) / w->time_step;
+    const double area = (1.0 - u) * area1 + u * area2;
+    double loss = init_area[de * w->naz() + az] * c / (area * init_c);
+    u = std::fabs(offset[1]) / w->de_seq.increment(de) - 1.0;
+    if (u > 4.0) loss *= 0.0;
+    u = std::fabs(offset[2]) / w->az_seq.increment(az) - 1.0;
+    if (u > 4.0) loss *= 0.0;
+    for (size_t f = 0; f < spread.size(); ++f) result[f] = loss;
+}
+
 // spreading_hybrid_gaussian::intensity (spreading_hybrid_gaussian.cc:65-119)
 void Spreading::intensity(double trho, double ttheta, double tphi, size_t de, size_t az,
                           const double offset[3], const double distance[3], double* result) {
+    if (classic) {
+        intensity_classic(trho, ttheta, tphi, de, az, offset, result);
+        return;
+    }
     double c;
     w->profile.sound_speed(trho, ttheta, tphi, w->R, &c, nullptr, nullptr, nullptr);
     for (size_t f = 0; f < spread.size(); ++f) spread[f] = kTwoPi * c / w->freq(f);

@@ -49,12 +49,23 @@ int main() {
                     (unsigned long long)collection.targetID(0, 0));
         const auto& curr = wave.curr();
         std::printf("curr %zu %zu %.17g\n", curr.num_de, curr.num_az, curr.pos_rho[curr.index(12, 4)]);
+        // the classic ray model (spreading_ray) on the same scenario
+        {
+            waveq3d::wave_queue classic(ocean, freq, pos, de, az, 0.1, &target, waveq3d::CLASSIC_RAY);
+            struct counter : waveq3d::eigenray_listener {
+                int n = 0;
+                void add_eigenray(size_t, size_t, waveq3d::eigenray_model::csptr, size_t) override { ++n; }
+            } c;
+            classic.add_eigenray_listener(&c);
+            while (classic.time() < 3.5) classic.step();
+            std::printf("classic %d\n", c.n);
+        }
         // an unsupported model is rejected with std::invalid_argument
         try {
-            waveq3d::wave_queue bad(ocean, freq, pos, de, az, 0.1, &target, waveq3d::CLASSIC_RAY);
-            std::printf("classic accepted\n");
+            waveq3d::wave_queue bad(ocean, freq, pos, de, az, 0.1, &target, (waveq3d::spreading_type)7);
+            std::printf("unknown accepted\n");
         } catch (const std::invalid_argument& e) {
-            std::printf("classic rejected\n");
+            std::printf("unknown rejected\n");
         }
     } catch (const std::exception& e) {
         std::printf("error %s\n", e.what());

@@ -1,7 +1,7 @@
 """The C++ wave_queue mirror (underseamodelinglibrary_amd/cpp) on the
 reference's eigenray_basic scenario (waveq3d/test/eigenray_test.cc:86-198):
-listener calls, check_eigenrays per step, snapshots and the rejection of an
-unsupported spreading model, compared with the oracle run of the same inputs.
+listener calls, check_eigenrays per step, snapshots, the classic ray model
+and the rejection of an unknown spreading model, compared with the oracle run of the same inputs.
 The CPU case links the program against the emulation build; the GPU case runs
 the program build() linked against libwq3d.so."""
 import os
@@ -16,6 +16,7 @@ from underseamodelinglibrary_amd import build, scenarios
 
 def _parse(out):
     rays, checks, classic, total, seq = [], None, None, None, None
+    unknown = None
     for line in out.splitlines():
         f = line.split()
         if f[0] == "total":
@@ -29,15 +30,19 @@ def _parse(out):
         elif f[0] == "checks":
             checks = int(f[1])
         elif f[0] == "classic":
-            classic = f[1]
+            classic = int(f[1])
+        elif f[0] == "unknown":
+            unknown = f[1]
         elif f[0] == "seq":
             seq = [float(x) for x in f[1:]]
-    return rays, checks, classic, total, seq
+    return rays, checks, (classic, unknown), total, seq
 
 
 def _check(out, oracle_mod):
-    rays, checks, classic, total, seq = _parse(out)
-    assert classic == "rejected"
+    rays, checks, (classic, unknown), total, seq = _parse(out)
+    # the classic ray model runs (3 eigenrays on this scenario); an unknown
+    # spreading type is refused with std::invalid_argument
+    assert classic == 3 and unknown == "rejected"
     # DE fan -60:5:60: index -1 and 1000 both clamp to the last element (60),
     # index 0 is the first (-60)
     assert seq == [60.0, 5.0, 60.0, -60.0]

@@ -176,6 +176,21 @@ def test_find_index_defined_for_every_input():
     assert checked > 100 and bad == 0
 
 
+def test_emulated_classic_ray_exact():
+    """CLASSIC_RAY spreading on config 1's ocean and on the branch-point fan
+    closing on itself in AZ, bit-identical to the oracle."""
+    out = _run("""
+        sc = scenarios.config1(de=seq_linear(-60.0, 3.0, 60.0), az=seq_linear(0.0, 20.0, 340.0),
+                               n_targets=(6, 6), time_max=40.0)
+        sc.spreading = _abi.SPREADING_CLASSIC_RAY
+        n = run(sc)
+        sc = scenarios.eigenray_branch_pt()
+        sc.spreading = _abi.SPREADING_CLASSIC_RAY
+        print("eigenrays", n + run(sc))
+    """)
+    assert int(out.split("eigenrays")[-1]) > 40
+
+
 def test_emulated_eigenverbs_volume_exact():
     out = _run("""
         sc = scenarios.config1(de=seq_linear(-60.0, 3.0, 60.0), az=seq_linear(0.0, 30.0, 330.0),
@@ -280,7 +295,8 @@ def test_emulated_tiny_and_ragged_fans_exact():
 def test_emulated_limits_and_errors():
     """The library's limits and argument checks: more than 4096 AZ rays or
     32 frequencies and CLASSIC_RAY spreading are refused as unsupported,
-    a non-positive time step and a negative step count as bad arguments;
+    an unknown spreading model as unsupported, a non-positive time step and
+    a negative step count as bad arguments;
     zero steps is a no-op.  Each failure leaves a message."""
     out = _run("""
         import ctypes as C
@@ -307,7 +323,10 @@ def test_emulated_limits_and_errors():
         zero_dt.time_step = 0.0
         print("wide", *status_of(wide))
         print("many", *status_of(many))
-        print("classic", *status_of(base, spreading=_abi.SPREADING_CLASSIC_RAY))
+        unknown = scenarios.config1(de=seq_linear(-10.0, 10.0, 3), az=seq_linear(0.0, 90.0, 4),
+                                    n_targets=(2, 2), time_max=1.0)
+        unknown.spreading = 7
+        print("classic", *status_of(unknown))
         print("zero_dt", *status_of(zero_dt))
         q = wave_queue(*base.args(), lib=lib)
         print("neg", lib.wq3d_step(q._fan, -1))
@@ -318,7 +337,7 @@ def test_emulated_limits_and_errors():
     lines = dict(l.split(" ", 1) for l in out.strip().splitlines() if " " in l)
     assert lines["wide"].startswith("2 ") and "4096" in lines["wide"]
     assert lines["many"].startswith("2 ") and "32" in lines["many"]
-    assert lines["classic"].startswith("2 ") and "HYBRID_GAUSSIAN" in lines["classic"]
+    assert lines["classic"].startswith("2 ") and "CLASSIC_RAY or HYBRID_GAUSSIAN" in lines["classic"]
     assert lines["zero_dt"].startswith("1 ") and "time_step" in lines["zero_dt"]
     assert lines["neg"] == "1"
     assert lines["zero"] == "0 0 0.0"

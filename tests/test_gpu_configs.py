@@ -216,3 +216,29 @@ def test_config3_size_fan_sane():
     assert r0.tobytes() == r1.tobytes() and i0.tobytes() == i1.tobytes()
     late_away, _ = _sane(r0, i0, p0, sc, steps, seam_az=(1, sc.az.size() - 2))
     assert late_away == 0
+
+
+@pytest.mark.parametrize("case", ["proploss_classic", "config1_thin", "branch_pt"])
+def test_classic_ray_parity(oracle_mod, case):
+    """CLASSIC_RAY spreading (spreading_ray.cc:47-130) against the oracle:
+    the reference's proploss_classic scenario (the oracle is pinned there to
+    spherical spreading within 0.2 dB), config 1's ocean on a thinned fan,
+    and the branch-point fan closing on itself in AZ (the lower-cell wrap)."""
+    if case == "proploss_classic":
+        sc = scenarios.proploss_classic()
+    elif case == "config1_thin":
+        sc = scenarios.config1(de=seq_linear(-60.0, 3.0, 60.0), az=seq_linear(0.0, 20.0, 340.0),
+                               n_targets=(6, 6), time_max=40.0)
+        sc.spreading = _abi.SPREADING_CLASSIC_RAY
+    else:
+        sc = scenarios.eigenray_branch_pt()
+        sc.spreading = _abi.SPREADING_CLASSIC_RAY
+    print(f"[classic] {case}: {sc.num_rays()} rays, {sc.num_targets()} targets, {sc.steps} steps", flush=True)
+    g = _gpu_run(sc, sc.steps)
+    o = _oracle_run(oracle_mod, sc, sc.steps)
+    gr, gi, gp = g.drain_eigenrays()
+    orr, oi, op = o.drain_eigenrays()
+    assert compare_eigenrays(gr, gi, gp, orr, oi, op, sc.freq.size()) >= 10
+    if case == "proploss_classic":
+        pl = 20.0 * np.log10(1500.0 * gr["travel_time"] - 2.0)
+        assert np.abs(gi.reshape(len(gr), -1) - pl[:, None]).max() < 0.2

@@ -187,3 +187,24 @@ def test_eigenray_branch_pt(oracle_mod):
             assert abs(s.source_de - 41.93623171) < 0.01
             assert abs(b.travel_time - 3.051676949) < 0.02
             assert abs(b.source_de + 60.91257162) < 1.0
+
+
+def test_proploss_classic(oracle_mod):
+    """waveq3d/test/proploss_test.cc:50-145: classic ray spreading
+    (spreading_ray) in an isovelocity ocean - every eigenray's transmission
+    loss, at every frequency, within 0.2 dB of spherical spreading,
+    20 log10(c t - 2)."""
+    sc = scenarios.proploss_classic()
+    q = oracle_mod.oracle_queue(*sc.args())
+    q._lib.orc_step(q._h, sc.steps)
+    recs, inten, _ = q.drain_eigenrays()
+    assert len(recs) == 10 and sorted(recs["t1"]) == list(range(10))
+    inten = inten.reshape(len(recs), -1)
+    pl = 20.0 * np.log10(1500.0 * recs["travel_time"] - 2.0)
+    assert np.abs(inten - pl[:, None]).max() < 0.2
+    # the hybrid Gaussian model on the same fan is a different estimate
+    sc.spreading = 1
+    q = oracle_mod.oracle_queue(*sc.args())
+    q._lib.orc_step(q._h, sc.steps)
+    _, hyb, _ = q.drain_eigenrays()
+    assert np.abs(hyb.reshape(10, -1) - inten).max() > 0.01

@@ -686,6 +686,7 @@ void emu_check_fan(const FanConst& F) {
     }
     emu_range(F.norm_de, F.nde, "F.norm_de");
     emu_range(F.norm_az, (int64_t)F.nde * F.naz, "F.norm_az");
+    emu_range(F.init_area, (int64_t)F.nde * F.naz, "F.init_area");
 }
 
 void emu_check_arrays(const FanConst& F, const Arrays& A) {
@@ -1427,8 +1428,9 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         return fail(WQ3D_ERR_ARGUMENT, "target arrays missing");
     if (nt > (1LL << 30)) return fail(WQ3D_ERR_UNSUPPORTED, "more than 2^30 targets");
     const bool has_spreading = d->de.size >= 3 && d->az.size >= 3;
-    if (has_spreading && d->spreading != WQ3D_SPREADING_HYBRID_GAUSSIAN)
-        return fail(WQ3D_ERR_UNSUPPORTED, "only HYBRID_GAUSSIAN spreading is supported");
+    if (has_spreading && d->spreading != WQ3D_SPREADING_HYBRID_GAUSSIAN &&
+        d->spreading != WQ3D_SPREADING_CLASSIC_RAY)
+        return fail(WQ3D_ERR_UNSUPPORTED, "unknown spreading model (CLASSIC_RAY or HYBRID_GAUSSIAN)");
 
     auto* f = new wq3d_fan();
     f->env = env;
@@ -1607,11 +1609,11 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
     }
     // spreading norms (spreading_hybrid_gaussian.cc:28-59)
     F.has_spreading = has_spreading;
+    F.spreading = d->spreading;
     {
         const int nde = F.nde, naz = F.naz;
-        std::vector<double> nde_v(nde, 0.0), naz_v((size_t)nde * naz, 0.0);
+        std::vector<double> nde_v(nde, 0.0), naz_v((size_t)nde * naz, 0.0), init_area((size_t)nde * naz, 0.0);
         if (has_spreading) {
-            std::vector<double> init_area((size_t)nde * naz, 0.0);
             for (int dd = 0; dd < nde - 1; ++dd) {
                 const double de1 = to_radians(hde.v[dd]);
                 const double de2 = to_radians(hde.v[dd + 1]);
@@ -1623,15 +1625,21 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
                     naz_v[(size_t)dd * naz + a] = init_area[(size_t)dd * naz + a] / nde_v[dd];
                 }
                 naz_v[(size_t)dd * naz + naz - 1] = naz_v[(size_t)dd * naz + naz - 2];
+                // spreading_ray.cc:35
+                init_area[(size_t)dd * naz + naz - 1] = init_area[(size_t)dd * naz + naz - 2];
             }
             nde_v[nde - 1] = nde_v[nde - 2];
-            for (int a = 0; a < naz - 1; ++a) naz_v[(size_t)(nde - 1) * naz + a] = naz_v[(size_t)(nde - 2) * naz + a];
+            for (int a = 0; a < naz - 1; ++a) {
+                naz_v[(size_t)(nde - 1) * naz + a] = naz_v[(size_t)(nde - 2) * naz + a];
+                init_area[(size_t)(nde - 1) * naz + a] = init_area[(size_t)(nde - 2) * naz + a];
+            }
             const double s = std::sqrt(2.0 * M_PI);
             for (auto& x : nde_v) x = x / s;
             for (auto& x : naz_v) x = x / s;
         }
         if (f->mem.upload(&F.norm_de, nde_v.data(), nde_v.size()) != hipSuccess ||
-            f->mem.upload(&F.norm_az, naz_v.data(), naz_v.size()) != hipSuccess)
+            f->mem.upload(&F.norm_az, naz_v.data(), naz_v.size()) != hipSuccess ||
+            f->mem.upload(&F.init_area, init_area.data(), init_area.size()) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "norm upload failed"));
     }
     const wq3d_thresholds& th = d->thresholds;
@@ -1704,6 +1712,10 @@ wq3d_status wq3d_fan_create(wq3d_env* env, const wq3d_fan_desc* d, wq3d_fan** ou
         });
         if (e != hipSuccess) return bail(fail(WQ3D_ERR_DEVICE, std::string("k_init: ") + hipGetErrorString(e)));
         if (rt::sync(f->stream) != hipSuccess) return bail(fail(WQ3D_ERR_DEVICE, "k_init failed"));
+        // spreading_ray.cc:41: the curr front's sound speed of ray (0, 0)
+        // once the queue is built
+        if (rt::memcpy(&F.init_c, f->A.c + (int64_t)S.curr * N, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            return bail(fail(WQ3D_ERR_DEVICE, "sound speed copy failed"));
     }
     *out = f;
     return WQ3D_OK;

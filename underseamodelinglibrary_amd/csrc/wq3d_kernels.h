@@ -94,6 +94,11 @@ struct FanConst {
     const double* norm_de;
     const double* norm_az;
     int32_t has_spreading;
+    // spreading_ray (CLASSIC_RAY, spreading_ray.cc:26-130): the model in use,
+    // the initial ray-tube areas and the sound speed at the source
+    int32_t spreading;          // wq3d_spreading
+    const double* init_area;    // [nde][naz]
+    double init_c;
     // thresholds
     double thr_intensity, thr_eigenverb;
     int32_t max_bottom, max_surface, max_caustic, max_upper, max_lower;
@@ -1907,6 +1912,43 @@ WQ_HD_INLINE void ndir_angles(const Arrays& A, int64_t k, double& de, double& az
     az = to_degrees(wqm::atan2(np, -nt));
 }
 
+// wvector1::area of four points of one slot (types/wvector1.cc:151-169):
+// the triangles (p1, p2, p3) and (p1, p3, p4) by Heron's formula, for the
+// cell p1 = (de, az), p2 = (de+1, az), p3 = (de+1, az+1), p4 = (de, az+1)
+WQ_HD_INLINE double cell_area(const BuildCtx& b, int slot, int de, int az) {
+    double a = ray_dist(b, slot, de, az, de + 1, az);
+    const double bb = ray_dist(b, slot, de, az, de + 1, az + 1);
+    double c = ray_dist(b, slot, de + 1, az, de + 1, az + 1);
+    const double area1 = 0.25 * sqrt((a + bb + c) * (bb + c - a) * (c + a - bb) * (a + bb - c));
+    a = ray_dist(b, slot, de, az, de, az + 1);
+    c = ray_dist(b, slot, de, az + 1, de + 1, az + 1);
+    const double area2 = 0.25 * sqrt((a + bb + c) * (bb + c - a) * (c + a - bb) * (a + bb - c));
+    return area1 + area2;
+}
+
+// spreading_ray::intensity (spreading_ray.cc:47-130): the ray tube's
+// initial area over its area now (interpolated in time toward prev or
+// next), times c(target) / c(source), zero more than four beam widths
+// outside the fan; the same for every frequency.  The reference steps to
+// the lower cell with a size_t `--az`, which at az = 0 of a fan closing on
+// itself indexes outside the front; here that cell is the fan's wrap,
+// naz - 2, as the hybrid model takes it.
+WQ_HD_INLINE double classic_spread(const BuildCtx& b, int de, int az, const double offset[3], double c_target) {
+    const FanConst& F = b.F;
+    if (offset[1] < 0.0) --de;
+    if (offset[2] < 0.0) az = (az > 0) ? az - 1 : F.naz - 2;
+    const double area1 = cell_area(b, b.S.curr, de, az);
+    const double area2 = cell_area(b, (offset[0] < 0.0) ? b.S.prev : b.S.next, de, az);
+    double u = fabs(offset[0]) / F.dt;
+    const double area = (1.0 - u) * area1 + u * area2;
+    double loss = F.init_area[(int64_t)de * F.naz + az] * c_target / (area * F.init_c);
+    u = fabs(offset[1]) / seq_inc(F.de, de) - 1.0;
+    if (u > 4.0) loss *= 0.0;
+    u = fabs(offset[2]) / seq_inc(F.az, az) - 1.0;
+    if (u > 4.0) loss *= 0.0;
+    return loss;
+}
+
 // is_closest_ray + build_eigenray (wave_queue.cc:411-702) for candidate c.
 // Split: called by the G lanes of a group with fsel = the lane's frequency;
 // every lane runs the same control flow (the walks stop on frequency 0,
@@ -1957,7 +1999,13 @@ WQ_HD_INLINE void body_build(int64_t c, int fsel, int G, double* lds, const DEnv
             const double w = (2.0 * M_PI) * ts_c / seq_value(F.freq, fs.freq(k));
             sp.spread[k] = w * w;
         }
-        {
+        if (F.spreading == WQ3D_SPREADING_CLASSIC_RAY) {
+            const double loss = classic_spread(b, de, az, offset, ts_c);
+            for (int k = 0; k < kSlots; ++k) {
+                sp.I_de[k] = loss;
+                sp.I_az[k] = 1.0;
+            }
+        } else {
             int a = az;
             double no[3] = {offset[0], offset[1], offset[2]};
             if (offset[2] < 0.0) {

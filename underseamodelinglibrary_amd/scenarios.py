@@ -45,6 +45,7 @@ class scenario:
     targets: Optional[object]
     steps: int
     notes: dict = field(default_factory=dict)
+    spreading: int = oc._abi.SPREADING_HYBRID_GAUSSIAN
 
     def num_rays(self):
         return self.de.size() * self.az.size()
@@ -53,7 +54,8 @@ class scenario:
         return 0 if self.targets is None else self.targets.size1() * self.targets.size2()
 
     def args(self):
-        return (self.ocean, self.freq, self.pos, self.de, self.az, self.time_step, self.targets)
+        return (self.ocean, self.freq, self.pos, self.de, self.az, self.time_step, self.targets,
+                self.spreading)
 
 
 def loop_steps(time_step, time_max):
@@ -264,6 +266,26 @@ def eigenray_branch_pt():
     target = wposition.from_arrays(lat, lng, alt)
     return scenario("eigenray_branch_pt", ocean, freq, pos, de, az, 0.1, target,
                     loop_steps(0.1, 3.5))
+
+
+def proploss_classic():
+    """waveq3d/test/proploss_test.cc:50-145 scenario: classic ray spreading
+    (CLASSIC_RAY) in an isovelocity ocean (1500 m/s, no absorption) with the
+    bottom 100 km down; source 45N 45W 15 km deep; DE -10.7:1:10, AZ
+    -10.5:2:10; 10, 100, 1000 and 10000 Hz; dt 0.1 s for 8 s; ten targets due
+    north at the source's depth, 0.02 to 0.11 deg away."""
+    compute_earth_radius(45.0)
+    profile = oc.profile_linear(1500.0, oc.attenuation_constant(0.0))
+    ocean = oc.ocean_model(oc.boundary_flat(), oc.boundary_flat(1e5), profile)
+    freq = seq_log(10.0, 10.0, 4)
+    pos = wposition1(45.0, -45.0, -15000.0)
+    de = seq_linear(-10.7, 1.0, 10.0)
+    az = seq_linear(-10.5, 2.0, 10.0)
+    lat = np.array([[45.0 + 0.01 * (n + 2.0)] for n in range(10)])
+    target = wposition.from_arrays(lat, np.full((10, 1), -45.0), np.full((10, 1), -15000.0))
+    sc = scenario("proploss_classic", ocean, freq, pos, de, az, 0.1, target, loop_steps(0.1, 8.0))
+    sc.spreading = oc._abi.SPREADING_CLASSIC_RAY
+    return sc
 
 
 def eigenray_basic():
