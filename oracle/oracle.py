@@ -53,6 +53,8 @@ def load():
     lib.orc_sum_eigenrays.restype = C.c_int32
     lib.orc_sum_eigenrays.argtypes = [C.c_void_p, dp, dp, i64, dp, C.c_int32, i64, i64, C.c_int32,
                                       C.POINTER(_abi.EigenrayTotal), dp, dp]
+    lib.orc_compute_offsets_core.restype = C.c_int32
+    lib.orc_compute_offsets_core.argtypes = [dp, dp, C.c_int32, dp, dp]
     lib.orc_grid_interp.restype = C.c_double
     lib.orc_grid_interp.argtypes = [C.POINTER(_abi.Grid), dp, dp]
     lib.orc_sound_speed.argtypes = [C.POINTER(_abi.ProfileDesc), C.c_double, C.c_double,

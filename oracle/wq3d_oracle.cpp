@@ -1385,6 +1385,36 @@ struct Queue {
                 }
             }
         }
+        if (offsets_core(D, delta, unstable, offset, distance) && distance[1] > 0.0) {
+            // the reference's else-branch (a positive DE remainder) takes its
+            // sign from the direction toward the next DE ray and the target
+            double cpa[3], nxt[3], prv[3], dir[3], speed;
+            collision_location(de, az, offset[0], cpa, dir, &speed);
+            collision_location(de + 1, az, offset[0], nxt, dir, &speed);
+            collision_location(de - 1, az, offset[0], prv, dir, &speed);
+            double nxt_dir[3], tgt_dir[3];
+            nxt_dir[0] = nxt[0] - cpa[0];
+            nxt_dir[1] = (nxt[1] - prv[1]) * cpa[0];
+            nxt_dir[2] = (nxt[2] - prv[2]) * cpa[0] * wqm::sin(cpa[1]);
+            if (std::fabs(nxt_dir[0]) < 0.01) nxt_dir[0] = 0.0;
+            tgt_dir[0] = tgt_r[t] - cpa[0];
+            tgt_dir[1] = (tgt_t[t] - cpa[1]) * cpa[0];
+            tgt_dir[2] = (tgt_p[t] - cpa[2]) * cpa[0] * wqm::sin(cpa[1]);
+            const double dot = inner3(nxt_dir, tgt_dir);
+            if (dot < 0.0) {
+                distance[1] *= -1.0;
+                offset[1] *= -1.0;
+            }
+        }
+    }
+
+    // compute_offsets (wave_queue.cc:708-924) from the neighbourhood's
+    // distances alone: make_taylor_coeff, the stability tests (determinant
+    // typo kept, wave_queue.cc:757-763), the 3 x 3 or 2 x 2 (time, AZ)
+    // solve, the per-axis distances and, when unstable, the DE distance as
+    // the remainder (its sign set by the caller).  Returns `unstable`.
+    static bool offsets_core(const double D[3][3][3], const double delta[3], bool unstable, double offset[3],
+                             double distance[3]) {
         double center, g[3], H[3][3], inv[3][3];
         make_taylor_coeff(D, delta, center, g, H);
         double determinant = smax(1e-6, H[0][0] * (H[1][1] * H[2][2] - H[1][2] * H[2][1]) +
@@ -1446,25 +1476,9 @@ struct Queue {
             } else {
                 distance[1] = std::sqrt(distance[1]);
                 offset[1] = std::fabs(offset[1]);
-                double cpa[3], nxt[3], prv[3], dir[3], speed;
-                collision_location(de, az, offset[0], cpa, dir, &speed);
-                collision_location(de + 1, az, offset[0], nxt, dir, &speed);
-                collision_location(de - 1, az, offset[0], prv, dir, &speed);
-                double nxt_dir[3], tgt_dir[3];
-                nxt_dir[0] = nxt[0] - cpa[0];
-                nxt_dir[1] = (nxt[1] - prv[1]) * cpa[0];
-                nxt_dir[2] = (nxt[2] - prv[2]) * cpa[0] * wqm::sin(cpa[1]);
-                if (std::fabs(nxt_dir[0]) < 0.01) nxt_dir[0] = 0.0;
-                tgt_dir[0] = tgt_r[t] - cpa[0];
-                tgt_dir[1] = (tgt_t[t] - cpa[1]) * cpa[0];
-                tgt_dir[2] = (tgt_p[t] - cpa[2]) * cpa[0] * wqm::sin(cpa[1]);
-                const double dot = inner3(nxt_dir, tgt_dir);
-                if (dot < 0.0) {
-                    distance[1] *= -1.0;
-                    offset[1] *= -1.0;
-                }
             }
         }
+        return unstable;
     }
 
     // wvector1::direction(double*, double*) (types/wvector1.cc:79-82)
@@ -2115,6 +2129,16 @@ void orc_reflect_loss(const wq3d_boundary_desc* d, double grazing, double* amp, 
     orc::Boundary b;
     b.load(*d);
     b.reflect_loss(grazing, 1, amp, phase);
+}
+
+/* The distance-only part of compute_offsets on a 3 x 3 x 3 neighbourhood
+ * (d27 in [time][DE][AZ] order): offsets, distances, and the returned
+ * unstable flag (pinned by exact quadratic neighbourhoods). */
+int32_t orc_compute_offsets_core(const double* d27, const double* delta, int32_t counters_differ,
+                                 double* offset, double* distance) {
+    double D[3][3][3];
+    for (int k = 0; k < 27; ++k) D[k / 9][(k / 3) % 3][k % 3] = d27[k];
+    return orc::Queue::offsets_core(D, delta, counters_differ != 0, offset, distance) ? 1 : 0;
 }
 
 /* Height and normal of a boundary descriptor (pins boundary_grid). */

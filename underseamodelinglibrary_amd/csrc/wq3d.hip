@@ -2237,6 +2237,16 @@ wq3d_status wq3d_kernel_times(wq3d_fan* f, double* ms, int64_t* launches, int32_
 }
 
 #ifdef WQ3D_EMULATE
+// Test hook of the CPU emulation build only: the distance-only part of the
+// kernels' compute_offsets on a 3 x 3 x 3 neighbourhood (d27 in [time][DE][AZ]
+// order); returns the unstable flag.
+int32_t wq3d_emu_compute_offsets_core(const double* d27, const double* delta, int32_t counters_differ,
+                                      double* offset, double* distance) {
+    double D[3][3][3];
+    for (int k = 0; k < 27; ++k) D[k / 9][(k / 3) % 3][k % 3] = d27[k];
+    return offsets_core(D, delta, counters_differ != 0, offset, distance) ? 1 : 0;
+}
+
 // Test hook of the CPU emulation build only: find_index of the kernels on a
 // host sequence, without the emulation's abort on non-finite arguments, so
 // a test can check the index for every input (NaN, infinities, values far

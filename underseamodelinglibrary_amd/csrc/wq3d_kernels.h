@@ -1399,10 +1399,14 @@ WQ_HD_INLINE bool counters_differ_at(const BuildCtx& b, int s, int64_t j, int s0
 // slots, has surface/bottom/caustic counts other than the centre's, or is on
 // an edge in the prev slot (the first stability test, wave_queue.cc:720-745;
 // see counters_differ_at)
-WQ_HD_INLINE void compute_offsets(const BuildCtx& b, int64_t t, int de, int az, const double D[3][3][3],
-                                const double delta[3], bool counters_differ, double offset[3],
-                                double distance[3]) {
-    const FanConst& F = b.F;
+// The part of compute_offsets that depends only on the neighbourhood's
+// distances: the Taylor coefficients, the stability tests, the 3 x 3 or
+// 2 x 2 (time, AZ) solve, the per-axis distances and, on the unstable
+// branch, the DE distance as the remainder.  Returns whether the branch was
+// unstable; the caller then gives a positive DE distance its sign from the
+// rays' geometry.
+WQ_HD_INLINE bool offsets_core(const double D[3][3][3], const double delta[3], bool counters_differ,
+                               double offset[3], double distance[3]) {
     bool unstable = counters_differ;
     double center, g[3], H[3][3], inv[3][3];
     make_taylor_coeff(D, delta, center, g, H);
@@ -1465,23 +1469,35 @@ WQ_HD_INLINE void compute_offsets(const BuildCtx& b, int64_t t, int de, int az, 
         } else {
             distance[1] = sqrt(distance[1]);
             offset[1] = fabs(offset[1]);
-            double cpa[3], nxt[3], prv[3];
-            collision_at(b, (int64_t)de * F.naz + az, offset[0], cpa);
-            collision_at(b, (int64_t)(de + 1) * F.naz + az, offset[0], nxt);
-            collision_at(b, (int64_t)(de - 1) * F.naz + az, offset[0], prv);
-            double nxt_dir[3], tgt_dir[3];
-            nxt_dir[0] = nxt[0] - cpa[0];
-            nxt_dir[1] = (nxt[1] - prv[1]) * cpa[0];
-            nxt_dir[2] = (nxt[2] - prv[2]) * cpa[0] * wqm::sin(cpa[1]);
-            if (fabs(nxt_dir[0]) < 0.01) nxt_dir[0] = 0.0;
-            tgt_dir[0] = F.tgt_r[t] - cpa[0];
-            tgt_dir[1] = (F.tgt_t[t] - cpa[1]) * cpa[0];
-            tgt_dir[2] = (F.tgt_p[t] - cpa[2]) * cpa[0] * wqm::sin(cpa[1]);
-            const double dot = inner3(nxt_dir, tgt_dir);
-            if (dot < 0.0) {
-                distance[1] *= -1.0;
-                offset[1] *= -1.0;
-            }
+        }
+    }
+    return unstable;
+}
+
+WQ_HD_INLINE void compute_offsets(const BuildCtx& b, int64_t t, int de, int az, const double D[3][3][3],
+                                const double delta[3], bool counters_differ, double offset[3],
+                                double distance[3]) {
+    const FanConst& F = b.F;
+    const bool unstable = offsets_core(D, delta, counters_differ, offset, distance);
+    // a positive remainder (the reference's else-branch) takes its sign
+    // from the direction toward the next DE ray and toward the target
+    if (unstable && distance[1] > 0.0) {
+        double cpa[3], nxt[3], prv[3];
+        collision_at(b, (int64_t)de * F.naz + az, offset[0], cpa);
+        collision_at(b, (int64_t)(de + 1) * F.naz + az, offset[0], nxt);
+        collision_at(b, (int64_t)(de - 1) * F.naz + az, offset[0], prv);
+        double nxt_dir[3], tgt_dir[3];
+        nxt_dir[0] = nxt[0] - cpa[0];
+        nxt_dir[1] = (nxt[1] - prv[1]) * cpa[0];
+        nxt_dir[2] = (nxt[2] - prv[2]) * cpa[0] * wqm::sin(cpa[1]);
+        if (fabs(nxt_dir[0]) < 0.01) nxt_dir[0] = 0.0;
+        tgt_dir[0] = F.tgt_r[t] - cpa[0];
+        tgt_dir[1] = (F.tgt_t[t] - cpa[1]) * cpa[0];
+        tgt_dir[2] = (F.tgt_p[t] - cpa[2]) * cpa[0] * wqm::sin(cpa[1]);
+        const double dot = inner3(nxt_dir, tgt_dir);
+        if (dot < 0.0) {
+            distance[1] *= -1.0;
+            offset[1] *= -1.0;
         }
     }
 }
