@@ -26,9 +26,13 @@ COMMON = textwrap.dedent("""
     lib = emulation.load()
     oracle.build(); oracle.load()
 
-    def run(sc, verbs=False):
+    def run(sc, verbs=False, limits=None):
         g = wave_queue(*sc.args(), lib=lib)
         o = oracle.oracle_queue(*sc.args())
+        if limits is not None:
+            for q in (g, o):
+                q.max_surface(limits[0])
+                q.max_bottom(limits[1])
         if verbs:
             d = type("L", (), {"add_eigenverb": lambda *a: None, "reflect": lambda *a: None})()
             for q in (g, o):
@@ -199,6 +203,29 @@ def test_emulated_eigenverbs_volume_exact():
         print("eigenrays", run(sc, verbs=True))
     """)
     assert int(out.split("eigenrays")[-1]) > 10
+
+
+def test_emulated_bounce_threshold_exact():
+    """waveq3d/test/reflection_test.cc:597-656 scenario with the reference's
+    limits (3 surface, 5 bottom bounces) and with the bottom alone limited to
+    2: eigenrays, eigenverbs and reflections bit-identical to the oracle."""
+    out = _run("""
+        n = run(scenarios.bounce_threshold(), verbs=True, limits=(3, 5))
+        print("eigenrays", n + run(scenarios.bounce_threshold(), verbs=True, limits=(999, 2)))
+    """)
+    assert int(out.split("eigenrays")[-1]) > 10
+
+
+def test_emulated_known_answer_scenarios_exact():
+    """Lloyd's mirror in range (1961 targets in one column) and depth on a
+    flat earth with tangent-spaced DE (proploss_test.cc:357-724) and the
+    hybrid Gaussian known-answer case (proploss_test.cc:153-245),
+    bit-identical to the oracle."""
+    out = _run("""
+        n = run(scenarios.lloyds_range()) + run(scenarios.lloyds_depth())
+        print("eigenrays", n + run(scenarios.proploss_gaussian()))
+    """)
+    assert int(out.split("eigenrays")[-1]) > 2000
 
 
 def test_emulated_az_wrap_fan_exact():

@@ -146,6 +146,36 @@ def test_eigenverb_and_reflection_parity(oracle_mod):
         assert (np.abs(gf[k] - of[k]) / scale).max() <= 1e-9, k
 
 
+@pytest.mark.parametrize("limits", [(3, 5), (999, 2)])
+def test_bounce_threshold_parity(oracle_mod, limits):
+    """waveq3d/test/reflection_test.cc:597-656 scenario with bounce limits:
+    the device drops exactly the oracle's eigenrays and eigenverbs."""
+    sc = scenarios.bounce_threshold()
+    print(f"[parity] {sc.name} limits {limits}: {sc.num_rays()} rays, {sc.steps} steps", flush=True)
+    g = wave_queue(*sc.args())
+    o = oracle_mod.oracle_queue(*sc.args())
+    dummy = type("L", (), {"add_eigenverb": lambda *a: None})()
+    for q in (g, o):
+        q.add_eigenverb_listener(dummy)
+        q.max_surface(limits[0])
+        q.max_bottom(limits[1])
+    g._lib.check(g._lib.wq3d_step(g._fan, sc.steps))
+    o._lib.orc_step(o._h, sc.steps)
+    n = _check_all(g, o, sc.freq.size())
+    assert n > 5
+    gv, gpw = g.drain_eigenverbs()
+    ov, opw = o.drain_eigenverbs()
+    assert len(gv) == len(ov) and len(ov) > 1000
+    assert gv["surface"].max() <= limits[0] and gv["bottom"].max() <= limits[1]
+    for k in ("step", "de", "az", "seq", "type", "surface", "bottom", "caustic", "upper", "lower"):
+        np.testing.assert_array_equal(gv[k], ov[k], err_msg=k)
+    for k in ("travel_time", "length", "width", "direction", "grazing", "sound_speed",
+              "pos_rho", "pos_theta", "pos_phi", "source_de", "source_az"):
+        scale = np.maximum(np.abs(ov[k]), 1.0)
+        assert (np.abs(gv[k] - ov[k]) / scale).max() <= 1e-9, k
+    assert (np.abs(gpw - opw) / np.maximum(np.abs(opw), 1e-300)).max() <= 1e-9
+
+
 def test_listener_replay_order(oracle_mod):
     """Listener calls arrive in the reference's order with identical content."""
     sc = scenarios.eigenray_basic()

@@ -29,8 +29,8 @@ from typing import Optional
 import numpy as np
 
 from .usml import ocean as oc
-from .usml.types import (compute_earth_radius, earth_radius, seq_linear, seq_log, set_earth_radius,
-                         to_colatitude, to_radians, wposition, wposition1)
+from .usml.types import (compute_earth_radius, earth_radius, seq_linear, seq_log, seq_rayfan,
+                         set_earth_radius, to_colatitude, to_radians, wposition, wposition1)
 
 
 @dataclass
@@ -285,6 +285,86 @@ def proploss_classic():
     target = wposition.from_arrays(lat, np.full((10, 1), -45.0), np.full((10, 1), -15000.0))
     sc = scenario("proploss_classic", ocean, freq, pos, de, az, 0.1, target, loop_steps(0.1, 8.0))
     sc.spreading = oc._abi.SPREADING_CLASSIC_RAY
+    return sc
+
+
+def bounce_threshold():
+    """waveq3d/test/reflection_test.cc:597-656 scenario: isovelocity 1500 m/s,
+    lossless flat 1000 m bottom, flat surface; source on the equator 500 m
+    deep; DE -90:1:90, AZ 0:30:360 (a fan closing on itself), 900 Hz; one
+    target 0.01 deg north at the source's depth; dt 0.1 s for 10 s.  The
+    reference test caps surface bounces at 3 and bottom bounces at 5."""
+    set_earth_radius(6371000.0)
+    profile = oc.profile_linear(1500.0)
+    ocean = oc.ocean_model(oc.boundary_flat(), oc.boundary_flat(1000.0, oc.reflect_loss_constant(0.0)),
+                           profile)
+    freq = seq_log(900.0, 1.0, 1)
+    pos = wposition1(0.0, 0.0, -500.0)
+    de = seq_linear(-90.0, 1.0, 90.0)
+    az = seq_linear(0.0, 30.0, 360.0)
+    target = wposition(1, 1, 0.01, 0.0, -500.0)
+    return scenario("bounce_threshold", ocean, freq, pos, de, az, 0.1, target, loop_steps(0.1, 10.0))
+
+
+def proploss_gaussian():
+    """waveq3d/test/proploss_test.cc:153-245 scenario: hybrid Gaussian beams
+    in an isovelocity ocean (1500 m/s, no absorption, flat 3000 m bottom);
+    source 45N 45W 1000 m deep; DE -90:1:90, AZ -5:1:5; 100-1000 Hz in
+    100 Hz steps; dt 0.1 s for 3.5 s; one target 0.02 deg north at the
+    source's depth (direct, surface and bottom paths)."""
+    compute_earth_radius(45.0)
+    profile = oc.profile_linear(1500.0, oc.attenuation_constant(0.0))
+    ocean = oc.ocean_model(oc.boundary_flat(), oc.boundary_flat(3000.0), profile)
+    freq = seq_linear(100.0, 100.0, 1000.0)
+    pos = wposition1(45.0, -45.0, -1000.0)
+    target = wposition(1, 1, 45.02, -45.0, -1000.0)
+    return scenario("proploss_gaussian", ocean, freq, pos, seq_linear(-90.0, 1.0, 90.0),
+                    seq_linear(-5.0, 1.0, 5.0), 0.1, target, loop_steps(0.1, 3.5))
+
+
+def _lloyds_ocean():
+    profile = oc.profile_linear(1500.0, oc.attenuation_constant(0.0))
+    profile.flat_earth(True)
+    return oc.ocean_model(oc.boundary_flat(), oc.boundary_flat(1e5), profile)
+
+
+def lloyds_range():
+    """waveq3d/test/proploss_test.cc:357-410 scenario: Lloyd's mirror on a
+    flat earth (isovelocity 1500 m/s, no absorption, bottom 100 km down);
+    source 45N 45W 25 m deep; DE 181 tangent-spaced rays, AZ -20:5:20;
+    500-2000 Hz in 500 Hz steps; dt 0.1 s for 8 s; 1961 targets 200 m deep
+    due north at ranges 200 m to 10 km in 5 m steps (one nautical mile per
+    arc minute of latitude).  The earth radius is the one the reference's
+    test suite leaves behind (its first case computes it at 45N)."""
+    compute_earth_radius(45.0)
+    ocean = _lloyds_ocean()
+    freq = seq_linear(500.0, 500.0, 4)
+    pos = wposition1(45.0, -45.0, -25.0)
+    ranges = seq_linear(200.0, 5.0, 10e3)
+    lat = np.array([[45.0 + ranges(n) / (1852.0 * 60.0)] for n in range(ranges.size())])
+    target = wposition.from_arrays(lat, np.full(lat.shape, -45.0), np.full(lat.shape, -200.0))
+    sc = scenario("lloyds_range", ocean, freq, pos, seq_rayfan(), seq_linear(-20.0, 5.0, 20.0), 0.1,
+                  target, loop_steps(0.1, 8.0))
+    sc.notes = {"ranges": np.array([ranges(n) for n in range(ranges.size())]), "src_alt": -25.0,
+                "trg_alt": -200.0}
+    return sc
+
+
+def lloyds_depth():
+    """waveq3d/test/proploss_test.cc:562-620 scenario: the Lloyd's mirror
+    ocean and fan of lloyds_range with 101 targets 10 km north of the
+    source at depths 0.1 to 50.1 m in 0.5 m steps."""
+    compute_earth_radius(45.0)
+    ocean = _lloyds_ocean()
+    freq = seq_linear(500.0, 500.0, 4)
+    pos = wposition1(45.0, -45.0, -25.0)
+    lat0 = 45.0 + math.degrees(10e3 / (earth_radius() - 25.0))
+    depths = seq_linear(-0.1, -0.5, -50.1)
+    alt = np.array([[depths(n)] for n in range(depths.size())])
+    target = wposition.from_arrays(np.full(alt.shape, lat0), np.full(alt.shape, -45.0), alt)
+    sc = scenario("lloyds_depth", ocean, freq, pos, seq_rayfan(), seq_linear(-20.0, 5.0, 20.0), 0.1,
+                  target, loop_steps(0.1, 8.0))
+    sc.notes = {"depths": alt[:, 0].copy(), "range": 10e3, "src_alt": -25.0}
     return sc
 
 
