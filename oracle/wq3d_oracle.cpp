@@ -20,24 +20,36 @@
  *   - listeners are replaced by record lists with explicit keys.
  *
  * Pinning (what checks this file: tests/test_oracle_known_answers.py,
- * tests/test_oracle_pins.py, tests/test_math.py, tests/test_collection.py):
+ * tests/test_oracle_pins.py, tests/test_oracle_offsets.py, tests/test_math.py,
+ * tests/test_collection.py):
  *   the reference's known answers for eigenray_basic, eigenray_concave and
  *   eigenray_branch_pt (travel times, angles, intensities, counts),
- *   reflect_flat_test bounce times, refraction_munk_range cycle ranges and
- *   vertex counts, refraction_isovelocity straight lines, eigenverb_accuracy
- *   lengths / widths / grazing angles, the source clamp, Thorp against the
- *   Generic Sonar Model table, constant attenuation, gen_grid linear /
- *   quadratic / derivative accuracy and exact trilinear reproduction; the
- *   Rayleigh loss against the textbook wavenumber form of the reflection
- *   coefficient and its critical-angle and normal-incidence limits; the
- *   shared elementary functions against glibc in ulps; the eigenray
- *   collection's 191.4 dB total (eigenrays/test/eigenrays_test.cc:57-88)
- *   and closed forms of the coherent / incoherent sums.  Parity unpinned:
- *   the unstable 2x2 / diagonal branches of compute_offsets, the hybrid
- *   Gaussian walk termination details, pchip end-point slopes on
- *   non-uniform axes, reflection_reinit numerics beyond bounce times and
- *   caustic counts on real folds, which no reference test constrains beyond
- *   loose tolerances.
+ *   proploss_classic, proploss_gaussian, Lloyd's mirror in range and depth
+ *   (path counts and types, bias / deviation / determination against the
+ *   image-source field), bounce_threshold (the limited run's records are
+ *   exactly the free run's within the limits), reflect_flat_test bounce
+ *   times, refraction_munk_range cycle ranges and vertex counts,
+ *   refraction_isovelocity straight lines, eigenverb_accuracy lengths /
+ *   widths / grazing angles, the source clamp, Thorp against the Generic
+ *   Sonar Model table, constant attenuation, gen_grid linear / quadratic /
+ *   derivative accuracy and exact trilinear reproduction.  Closed forms:
+ *   compute_offsets on neighbourhoods sampled from exact quadratics (the
+ *   minimum, the determinant typo, the 2x2 and clipped branches); rays
+ *   between flat boundaries following their image-source chains through
+ *   the reflection restart (within 1 cm over 16 bounces); the phase of
+ *   every ray and eigenray as -pi per surface bounce, the bottom loss phase
+ *   per bottom bounce and -pi/2 per caustic; pchip interior slopes against
+ *   Moler's weighted harmonic mean and its end slopes against the exact
+ *   three-point formula with the reference's (2 + h1 + h2) weight; the
+ *   Rayleigh loss against the textbook reflection coefficient; the shared
+ *   elementary functions against glibc in ulps; the eigenray collection's
+ *   191.4 dB total (eigenrays/test/eigenrays_test.cc:57-88) and closed
+ *   forms of the coherent / incoherent sums.  Parity unpinned: the hybrid
+ *   Gaussian walk's stopping ratio (1.002305238) - moving it to 1.1 changes
+ *   the pinned scenarios by less than their tolerances - the fold test that
+ *   counts caustics (pinned only through the phase it adds, caustics
+ *   occurring in the Munk channel and none on Lloyd's mirror paths) and the
+ *   geometric DE sign on compute_offsets' unstable branch.
  */
 #include <algorithm>
 #include <cmath>

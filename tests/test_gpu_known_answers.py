@@ -12,13 +12,16 @@ oracle's on the same inputs:
   285-289);
 * Munk cycle ranges and vertex counts on a flat earth (refraction_test.cc:
   827-955), which exercises the flat-earth sound speed correction
-  (ocean/profile_model.cc:14-26) on the device."""
+  (ocean/profile_model.cc:14-26) on the device;
+* rays bouncing between a flat surface and a flat bottom following their
+  image-source chains of straight lines (reflection_model.cc:33-360)."""
 import pytest
 
 from underseamodelinglibrary_amd import _lib, scenarios
 from underseamodelinglibrary_amd.usml.waveq3d import wave_queue
 
 import test_oracle_known_answers as ka
+import test_oracle_pins as pins
 from parity import compare_eigenrays
 
 pytestmark = pytest.mark.gpu
@@ -70,3 +73,8 @@ def test_caustic_and_reflection_phase_gpu():
 def test_refraction_munk_range_flat_earth_gpu():
     print("[known answer] refraction_munk_range on a flat earth: 29 rays, 950 steps", flush=True)
     ka.munk_range_check(lambda *a: wave_queue(*a))
+
+
+def test_reflection_reinit_image_source_gpu():
+    print("[known answer] image-source chains: 21 rays, 120 steps", flush=True)
+    pins.image_source_check(lambda *a: wave_queue(*a))

@@ -389,3 +389,83 @@ def test_refraction_isovelocity(oracle_mod):
         q.step()
         steps += 1
     assert steps > 100
+
+
+def image_source_path(S, d, R_surface, R_bottom, s_max):
+    """Straight legs of a ray from S along unit d between two concentric
+    spheres (an isovelocity ocean between a flat surface and a flat bottom
+    on a round earth), each reflection a mirror image about the sphere's
+    normal at the hit point.  Returns [(path length at leg start, start
+    point, unit direction)]."""
+    legs = [(0.0, S.copy(), d.copy())]
+    P, s = S.copy(), 0.0
+    while s < s_max:
+        b = P @ d
+        if b < 0.0:
+            u = -b - math.sqrt(b * b - (P @ P - R_bottom * R_bottom))
+        else:
+            u = -b + math.sqrt(b * b - (P @ P - R_surface * R_surface))
+        P = P + u * d
+        s += u
+        n = P / np.linalg.norm(P)
+        d = d - 2.0 * (d @ n) * n
+        legs.append((s, P.copy(), d.copy()))
+    return legs
+
+
+def test_reflection_reinit_image_source(oracle_mod):
+    """reflection_model.cc:33-360 (bounce, refined collision point and the
+    reflection_reinit restart of all four wavefronts): in an isovelocity
+    ocean between a flat surface and a flat 1000 m bottom on a round earth,
+    every ray is a chain of straight lines mirrored at each boundary (the
+    image-source construction).  21 rays (DE -67.3:9.1:-12.7, AZ 21.7:40:
+    101.7) from 137 m deep, followed for 12 s through up to 16 reflections,
+    stay within 1 cm of that chain at every step, and the restart keeps
+    |xi| c = 1 (the slowness vector's length) to rounding.  A restart that
+    left the current or previous wavefront stale bends the path by metres
+    (the restarted past wavefront is dropped by the rotation that follows)."""
+    image_source_check(oracle_mod.oracle_queue)
+
+
+def image_source_check(make_queue):
+    compute_earth_radius(45.0)
+    R = earth_radius()
+    ocean = oc.ocean_model(oc.boundary_flat(), oc.boundary_flat(1000.0),
+                           oc.profile_linear(1500.0, oc.attenuation_constant(0.0)))
+    lat0, lng0, z0 = 44.7, -45.3, -137.0
+    des = seq_linear(-67.3, 9.1, 7)
+    azs = seq_linear(21.7, 40.0, 3)
+    q = make_queue(ocean, seq_log(1000.0, 1.0, 1), wposition1(lat0, lng0, z0), des, azs, 0.1)
+
+    def cart(r, t, p):
+        return np.array([r * math.sin(t) * math.cos(p), r * math.sin(t) * math.sin(p), r * math.cos(t)])
+
+    th0, ph0 = math.radians(90.0 - lat0), math.radians(lng0)
+    S = cart(R + z0, th0, ph0)
+    up = np.array([math.sin(th0) * math.cos(ph0), math.sin(th0) * math.sin(ph0), math.cos(th0)])
+    south = np.array([math.cos(th0) * math.cos(ph0), math.cos(th0) * math.sin(ph0), -math.sin(th0)])
+    east = np.array([-math.sin(ph0), math.cos(ph0), 0.0])
+    T = 12.0
+    legs = {}
+    for i in range(des.size()):
+        for j in range(azs.size()):
+            de, az = math.radians(des(i)), math.radians(azs(j))
+            d = math.sin(de) * up - math.cos(de) * math.cos(az) * south + math.cos(de) * math.sin(az) * east
+            legs[i, j] = image_source_path(S, d, R, R - 1000.0, 1500.0 * T + 10.0)
+    worst = worst_xi = 0.0
+    bounces = 0
+    while q.time() < T:
+        q.step()
+        c = q.curr()
+        s = 1500.0 * q.time()
+        for (i, j), chain in legs.items():
+            s0, P, d = next(leg for leg in reversed(chain) if s >= leg[0])
+            pos = cart(c.position.rho(i, j), c.position.theta(i, j), c.position.phi(i, j))
+            worst = max(worst, float(np.linalg.norm(pos - (P + (s - s0) * d))))
+            xi = math.sqrt(c.ndirection.rho(i, j) ** 2 + c.ndirection.theta(i, j) ** 2 +
+                           c.ndirection.phi(i, j) ** 2)
+            worst_xi = max(worst_xi, abs(xi * c.sound_speed(i, j) - 1.0))
+            bounces = max(bounces, c.surface(i, j) + c.bottom(i, j))
+    assert worst < 0.01
+    assert worst_xi < 1e-13
+    assert bounces >= 16
