@@ -53,6 +53,9 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxBatchFans = 1024;  // fans of one batch (wq3d_batch_create)
 constexpr int kBuildLanes = 32;  // lanes per eigenray candidate in k_build
+constexpr int kBuildLds = 4 * 27;  // LDS doubles per candidate group: two cubes of 2 x 27 values
+static_assert(kMaxFreq <= kBuildLanes && kBlock % kBuildLanes == 0 && 64 % kBuildLanes == 0,
+              "a candidate group is kBuildLanes lanes of one wavefront, one lane per frequency");
 // k_detect: ray tiles of kTile x kTile (AZ x DE, the tiles of the window
 // records), one ray per lane, and hit keys buffered per wavefront before a
 // flush
@@ -414,11 +417,12 @@ __global__ void __launch_bounds__(kBlock) k_build(const KParams* __restrict__ P,
     const Records& Rc = P->R;
     const unsigned long long raw = *Rc.cand_count;
     const int64_t ncand = (int64_t)(raw < (unsigned long long)Rc.cand_cap ? raw : (unsigned long long)Rc.cand_cap);
-    // 27 neighbourhood values (or 2 x 27) per group of at least 4 lanes
-    __shared__ double s_cube[(kBlock / 4) * 54];
+    // two neighbourhood cubes of 2 x 27 values per group (groups have at
+    // least kBuildLanes lanes)
+    __shared__ double s_cube[(kBlock / kBuildLanes) * kBuildLds];
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane_in_group = (int)(gid % group);
-    double* lds = group >= 4 ? s_cube + (threadIdx.x / group) * 54 : nullptr;
+    double* lds = s_cube + (threadIdx.x / group) * kBuildLds;
     for (int64_t c = gid / group; c < ncand; c += (int64_t)gridDim.x * blockDim.x / group)
         body_build<true>(c, lane_in_group, group, lds, P->E, P->F, P->A, S, Rc, time, step_key);
 }
@@ -508,7 +512,7 @@ __global__ void __launch_bounds__(kBlock) k_build_multi(const KParams* const* __
                                                        const int32_t* __restrict__ ids, int n, Slots S,
                                                        const FanClock* __restrict__ clk, int group) {
     __shared__ int64_t s_first[kMaxBatchFans + 1];
-    __shared__ double s_cube[(kBlock / 4) * 54];
+    __shared__ double s_cube[(kBlock / kBuildLanes) * kBuildLds];
     for (int g = threadIdx.x; g < n; g += blockDim.x) {
         const Records& R = kps[ids[g]]->R;
         const unsigned long long raw = *R.cand_count;
@@ -523,7 +527,7 @@ __global__ void __launch_bounds__(kBlock) k_build_multi(const KParams* const* __
     const int64_t total = s_first[n];
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane_in_group = (int)(gid % group);
-    double* lds = s_cube + (threadIdx.x / group) * 54;
+    double* lds = s_cube + (threadIdx.x / group) * kBuildLds;
     for (int64_t c = gid / group; c < total; c += (int64_t)gridDim.x * blockDim.x / group) {
         const int g = fan_of_block(s_first, n, c);
         const int f = ids[g];
@@ -915,14 +919,14 @@ void launch_build(hipStream_t st, const KParams* kp, const DEnv& E, const FanCon
     while (group < P->F.nf) group <<= 1;
     emu::Group grp;
     grp.G = group;
-    std::vector<double> lds(54, 0.0);
+    std::vector<double> lds(kBuildLds, 0.0);
     auto lane_main = [&](int g) {
         emu::LaneState& ls = emu::lane_state();
         ls.group = &grp;
         ls.lane = g;
         for (int64_t c = 0; c < n; ++c) {
             ls.tag = c;
-            body_build<true>(c, g, group, group >= 4 ? lds.data() : nullptr, P->E, P->F, P->A, S, Rc, t, key);
+            body_build<true>(c, g, group, lds.data(), P->E, P->F, P->A, S, Rc, t, key);
             grp.barrier(g, -1, c);  // the next candidate starts together
         }
         ls.group = nullptr;

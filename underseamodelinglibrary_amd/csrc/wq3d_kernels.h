@@ -1713,21 +1713,24 @@ WQ_HD_INLINE void wave_sync(int where = __builtin_LINE()) {
 }
 
 // Values at the 27 points k = s*9 + nde*3 + naz of a (time, DE, AZ)
-// neighbourhood.  A split build with an LDS area computes them a lane at a
-// time (lane g takes k = g, g + L, ...) and every lane reads all 27 back;
-// otherwise one thread computes them all.  fn(k, out, stride) writes out[0]
-// and, for two values per point, out[stride].
+// neighbourhood, NVal per point (value v of point k at [v * 27 + k]).  A
+// split build with an LDS area computes them a lane at a time (lane g takes
+// k = g, g + L, ...) into that area, and every lane then reads them there:
+// the returned pointer is the LDS area, which holds them until the group's
+// next fill_cube on it (the group's lanes run in one wavefront, so program
+// order and the wave barriers order one candidate's reads before the next
+// candidate's writes).  Otherwise one thread computes them all into `out`,
+// which is returned.
 template <int NVal, bool Split, class PointFn>
-WQ_HD_INLINE void fill_cube(const FreqSel<Split>& fs, double* lds, double* out, PointFn fn) {
-    if (Split && lds != nullptr) {
+WQ_HD_INLINE const double* fill_cube(const FreqSel<Split>& fs, double* lds, double* out, PointFn fn) {
+    if (Split) {  // a split build always has its group's LDS area
+        wave_sync();
         for (int k = fs.fsel; k < 27; k += fs.lanes()) fn(k, lds + k, 27);
         wave_sync();
-#pragma unroll
-        for (int k = 0; k < 27 * NVal; ++k) out[k] = lds[k];
-        wave_sync();
-    } else {
-        for (int k = 0; k < 27; ++k) fn(k, out + k, 27);
+        return lds;
     }
+    for (int k = 0; k < 27; ++k) fn(k, out + k, 27);
+    return out;
 }
 
 // spreading_hybrid_gaussian::gaussian (.h:148-153), accumulated into out
@@ -1988,13 +1991,16 @@ WQ_HD_INLINE void body_build(int64_t c, int fsel, int G, double* lds, const DEnv
         const int64_t ic0 = at(S.curr, F.N, i);
         const int s0 = A.srf[ic0], b0 = A.btm[ic0], c0 = A.cst[ic0];
         bool differ = false;
-        double DE[2][3][3][3];
-        fill_cube<2>(fs, lds, &DE[0][0][0][0], [&](int k, double* out, int stride) {
-            const int64_t j = cube_ray(F, de, az, (k / 3) % 3, k % 3);
-            out[0] = dist2_slot(b, sl[k / 9], j, t);
-            out[stride] = (k / 9 == 1 && A.edge[at(S.curr, F.N, j)] != 0) ? 1.0 : 0.0;
-            differ = differ || counters_differ_at(b, k / 9, j, s0, b0, c0);
-        });
+        // the split build keeps both cubes in its group's LDS area (2 x 54
+        // values), the serial build in these private arrays
+        double DE_own[Split ? 1 : 2 * 27];
+        const double(*DE)[3][3][3] = reinterpret_cast<const double(*)[3][3][3]>(
+            fill_cube<2>(fs, lds, DE_own, [&](int k, double* out, int stride) {
+                const int64_t j = cube_ray(F, de, az, (k / 3) % 3, k % 3);
+                out[0] = dist2_slot(b, sl[k / 9], j, t);
+                out[stride] = (k / 9 == 1 && A.edge[at(S.curr, F.N, j)] != 0) ? 1.0 : 0.0;
+                differ = differ || counters_differ_at(b, k / 9, j, s0, b0, c0);
+            }));
         if (Split) differ = group_any(differ, G);
         const double(&D)[3][3][3] = DE[0];
         // the time-CPA pre-test (wave_queue.cc:384-394), then is_closest_ray
@@ -2063,10 +2069,12 @@ WQ_HD_INLINE void body_build(int64_t c, int fsel, int G, double* lds, const DEnv
         }
         if (Split) above = group_any(above, G);
         if (!above) return;
-        double VW[2][3][3][3];  // launch-angle DE and AZ of the neighbourhood's directions
-        fill_cube<2>(fs, lds, &VW[0][0][0][0], [&](int k, double* out, int stride) {
-            ndir_angles(A, at(sl[k / 9], F.N, cube_ray(F, de, az, (k / 3) % 3, k % 3)), out[0], out[stride]);
-        });
+        // launch-angle DE and AZ of the neighbourhood's directions
+        double VW_own[Split ? 1 : 2 * 27];
+        const double(*VW)[3][3][3] = reinterpret_cast<const double(*)[3][3][3]>(
+            fill_cube<2>(fs, Split ? lds + 2 * 27 : nullptr, VW_own, [&](int k, double* out, int stride) {
+                ndir_angles(A, at(sl[k / 9], F.N, cube_ray(F, de, az, (k / 3) % 3, k % 3)), out[0], out[stride]);
+            }));
         const double(&V)[3][3][3] = VW[0];
         const double(&W)[3][3][3] = VW[1];
         double cen, g[3], H[3][3], Ho[3];
