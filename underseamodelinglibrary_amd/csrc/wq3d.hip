@@ -412,7 +412,11 @@ __global__ void __launch_bounds__(kBlock) k_sort_lists(const int64_t* offset, in
 // is_closest_ray + build_eigenray per candidate, grid-stride over the
 // device-side candidate count; each candidate is built by a group of G
 // consecutive lanes, one per frequency (body_build<true>)
-__global__ void __launch_bounds__(kBlock) k_build(const KParams* __restrict__ P, Slots S, double time,
+// Two wavefronts per SIMD: uncapped, construction takes 256 VGPRs plus AGPRs
+// and runs one wavefront per SIMD; the cap spills about 44 VGPRs to scratch,
+// but the second wavefront hides more of each candidate's dependent chains
+// (k_build_multi on config 5's 32 fans: 177 -> 131 us per step).
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) k_build(const KParams* __restrict__ P, Slots S, double time,
                                                  int64_t step_key, int group) {
     const Records& Rc = P->R;
     const unsigned long long raw = *Rc.cand_count;
@@ -495,7 +499,10 @@ __global__ void __launch_bounds__(kBlock) k_supmask_multi(const KParams* const* 
 
 // each fan's range is a whole detect grid, a multiple of kXcds blocks, so a
 // block's die (its index mod kXcds) is the one the single-fan grid gives it
-__global__ void __launch_bounds__(kBlock) k_detect_multi(const KParams* const* __restrict__ kps,
+// Three wavefronts per SIMD, as the single-fan k_detect gets uncapped: the
+// fan lookup raises the fused kernel to 178 VGPRs (two wavefronts); the cap
+// spills 24 of them, and the search stage is 11 % faster.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) k_detect_multi(const KParams* const* __restrict__ kps,
                                                         const int32_t* __restrict__ ids,
                                                         const int64_t* __restrict__ first, int n, Slots S) {
     const int g = fan_of_block(first, n, blockIdx.x);
@@ -508,7 +515,8 @@ __global__ void __launch_bounds__(kBlock) k_detect_multi(const KParams* const* _
 // the candidates of every fan as one list: a block reads the fans' counts,
 // and each group of `group` lanes takes candidates of the whole list
 
-__global__ void __launch_bounds__(kBlock) k_build_multi(const KParams* const* __restrict__ kps,
+// two wavefronts per SIMD, as k_build
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) k_build_multi(const KParams* const* __restrict__ kps,
                                                        const int32_t* __restrict__ ids, int n, Slots S,
                                                        const FanClock* __restrict__ clk, int group) {
     __shared__ int64_t s_first[kMaxBatchFans + 1];
