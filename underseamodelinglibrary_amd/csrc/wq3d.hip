@@ -1,8 +1,9 @@
 // wq3d.hip - HIP kernels (gfx950, FP64) and the C ABI of libwq3d.so.
 //
-// One wave_queue::step() (waveq3d/wave_queue.cc:155-192) is five launches on
+// One wave_queue::step() (waveq3d/wave_queue.cc:155-192) is six launches on
 // the fan's stream:
-//   k_bounce   detect_reflections without find_edges (wave_queue.cc:197-292):
+//   k_reflect_light, k_bounce
+//              detect_reflections without find_edges (wave_queue.cc:197-292):
 //              one thread per ray for the caustic and vertex counters, then
 //              the reflections of the rays flag[] marks
 //              (reflection_model.cc:33-360).  The reference's de-major
@@ -99,14 +100,17 @@ __global__ void __launch_bounds__(kBlock) k_init(DEnv E, FanConst F, Arrays A, S
     if (i < F.N) body_init<KindsOf<PK, FLAT>>(i, E, F, A, S);
 }
 
-// detect_reflections: every ray's caustic and vertex counters, then the
-// reflecting rays (flag[] != 0; volume scattering needs every ray).  Rows of
-// the fan reflect together, so most wavefronts leave after the light part.
-template <int PK, bool FLAT>
-__global__ void __launch_bounds__(kBlock) k_bounce(const KParams* __restrict__ P, Slots S, double time,
-                                                  int64_t step_key) {
+// detect_reflections in two launches.  k_reflect_light: every ray's caustic
+// test against its lower DE neighbour and its vertex counters (a few loads,
+// few registers, so the launch runs at full occupancy); it also clears the
+// step's eigenray search counters (read by later launches).  k_bounce: the
+// reflecting rays (flag[] != 0; volume scattering needs every ray), whose
+// reflection chain needs about 250 VGPRs; rows of the fan reflect together,
+// so most of its wavefronts leave after reading their flags.  A ray's light
+// part reads its neighbour only when that neighbour does not reflect, so the
+// split gives the results of one thread doing both parts.
+__global__ void __launch_bounds__(kBlock) k_reflect_light(const KParams* __restrict__ P, Slots S) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // the eigenray search counters of this step (read by later launches)
     if (i == 0) {
         *P->R.cand_count = 0ull;
         *P->R.hit_count = 0ull;
@@ -114,6 +118,13 @@ __global__ void __launch_bounds__(kBlock) k_bounce(const KParams* __restrict__ P
     }
     if (i >= P->F.N) return;
     body_reflect_light(i, P->F, P->A, S);
+}
+
+template <int PK, bool FLAT>
+__global__ void __launch_bounds__(kBlock) k_bounce(const KParams* __restrict__ P, Slots S, double time,
+                                                  int64_t step_key) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P->F.N) return;
     if (P->A.flag[i] == 0 && !(P->F.want_verbs && P->E.nvol > 0)) return;
     body_bounce<KindsOf<PK, FLAT>>(i, P->E, P->F, P->A, S, P->R, time, step_key);
 }
@@ -450,6 +461,22 @@ __device__ __forceinline__ int fan_of_block(const int64_t* __restrict__ first, i
     return lo;
 }
 
+__global__ void __launch_bounds__(kBlock) k_reflect_light_multi(const KParams* const* __restrict__ kps,
+                                                               const int32_t* __restrict__ ids,
+                                                               const int64_t* __restrict__ first, int n,
+                                                               Slots S) {
+    const int g = fan_of_block(first, n, blockIdx.x);
+    const KParams* P = kps[ids[g]];
+    const int64_t i = (int64_t)(blockIdx.x - first[g]) * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *P->R.cand_count = 0ull;
+        *P->R.hit_count = 0ull;
+        *P->R.pass_count = 0ull;
+    }
+    if (i >= P->F.N) return;
+    body_reflect_light(i, P->F, P->A, S);
+}
+
 template <int PK, bool FLAT>
 __global__ void __launch_bounds__(kBlock) k_bounce_multi(const KParams* const* __restrict__ kps,
                                                         const int32_t* __restrict__ ids,
@@ -459,13 +486,7 @@ __global__ void __launch_bounds__(kBlock) k_bounce_multi(const KParams* const* _
     const int f = ids[g];
     const KParams* P = kps[f];
     const int64_t i = (int64_t)(blockIdx.x - first[g]) * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        *P->R.cand_count = 0ull;
-        *P->R.hit_count = 0ull;
-        *P->R.pass_count = 0ull;
-    }
     if (i >= P->F.N) return;
-    body_reflect_light(i, P->F, P->A, S);
     if (P->A.flag[i] == 0 && !(P->F.want_verbs && P->E.nvol > 0)) return;
     body_bounce<KindsOf<PK, FLAT>>(i, P->E, P->F, P->A, S, P->R, clk[f].time, clk[f].step);
 }
@@ -791,6 +812,7 @@ void launch_init(hipStream_t st, unsigned grid, const DEnv& E, const FanConst& F
 void launch_bounce(hipStream_t st, unsigned grid, const KParams* kp, const DEnv& E, const FanConst& F,
                    const Arrays& A, const Slots& S, const Records& R, double t, int64_t key) {
 #ifndef WQ3D_EMULATE
+    hipLaunchKernelGGL(k_reflect_light, dim3(grid), dim3(kBlock), 0, st, kp, S);
     switch (env_variant(E)) {
     case 1:
         hipLaunchKernelGGL((k_bounce<WQ3D_PROFILE_MUNK, true>), dim3(grid), dim3(kBlock), 0, st, kp, S, t, key);

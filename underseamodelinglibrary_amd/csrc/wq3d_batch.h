@@ -94,6 +94,9 @@ wq3d_status batch_launch_step(wq3d_batch* b, const Slots& S, int rset) {
     const KParams* const* kps = b->kps + (size_t)rset * n;
 #ifndef WQ3D_EMULATE
     const unsigned tpb = (unsigned)b->shape.ray_threads;
+    for (const auto& g : b->groups)
+        hipLaunchKernelGGL(k_reflect_light_multi, dim3((unsigned)g.blocks), dim3(tpb), 0, b->stream, kps, g.d_ids,
+                           g.d_first, (int)g.ids.size(), S);
     for (const auto& g : b->groups) {
         const int ng = (int)g.ids.size();
         switch (g.variant) {
