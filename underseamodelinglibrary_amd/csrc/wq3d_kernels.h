@@ -51,7 +51,7 @@ struct Arrays {
     int32_t *srf, *btm, *cst, *upr, *lwr;
     uint8_t* edge;
     uint8_t* flag;              // [N] reflection kind for the coming step
-    double* win;                // [N][16] eigenray pre-test window of the step (store_window)
+    double* win;                // eigenray pre-test window of the step, tile by tile (store_window)
     uint32_t* tsup;             // [tiles][sup_words(F)] bit u: some ray of the tile passes super-block u's window bound
 };
 
@@ -1097,14 +1097,16 @@ WQ_HD_INLINE bool detect_ray_window(const FanConst& F, const Arrays& A, const Sl
     return true;
 }
 
-// The candidate region of ray i for the current step, computed once by
-// k_advance (after the new next slot's update()) and read by every target
-// group of k_detect.  Record of kWinStride doubles: {form.k, form.a[6], w,
-// prev.k, prev.a[6], 1 + stat bits (0: ray not searched), nb[n].k,
-// nb[n].a[6] for n = 0..3}.  The neighbours' points are the curr slot's,
-// which k_advance does not write; their edge flags are written by the same
-// launch, so the reader applies them (load_cell).
-constexpr int kWinStride = 44;
+// The pre-test window of ray i for the current step, computed once by
+// k_advance (after the new next slot's update()) and read by k_supmask and
+// by every target group of k_detect.  Record of kWinStride doubles:
+// {form.k, form.a[6], w}; w = -inf marks a ray that is not searched.  The
+// rest of the candidate region - the previous-time form and the neighbour
+// forms - k_detect computes from the slots' distance snapshots when a
+// wavefront needs it (load_cell_rest): the same points through the same
+// lin_diff give the same forms, and the record stays a quarter of the size
+// (k_advance writes 64 B per ray instead of 352).
+constexpr int kWinStride = 8;
 // Records are stored tile by tile (kWinTile x kWinTile rays, AZ fastest,
 // the k_detect wavefront's tile) and field-major within a tile, so that a
 // wavefront reads each field of its 64 rays as one contiguous 512-byte row.
@@ -1130,8 +1132,7 @@ WQ_HD_INLINE void store_window(const FanConst& F, const Arrays& A, const Slots& 
     double* out = A.win + win_slot(F, i);
     DetectWindow W;
     if (F.nt <= 0 || !detect_ray_window(F, A, S, i, W)) {
-        out[7 * kWinField] = -HUGE_VAL;  // the width field marks the ray as not searched too
-        out[15 * kWinField] = 0.0;
+        out[7 * kWinField] = -HUGE_VAL;  // the width field marks the ray as not searched
         return;
     }
     out[0] = W.form.k;
@@ -1139,30 +1140,14 @@ WQ_HD_INLINE void store_window(const FanConst& F, const Arrays& A, const Slots& 
     // a searched ray's width is never -inf (no window test tells -inf from
     // the most negative finite width)
     out[7 * kWinField] = W.w == -HUGE_VAL ? -0x1.fffffffffffffp+1023 : W.w;
-    out[8 * kWinField] = W.prev.k;
-    for (int j = 0; j < 6; ++j) out[(9 + j) * kWinField] = W.prev.a[j];
-    int32_t nbj[4];
-    uint32_t stat, upper;
-    ray_neighbours(F, i, nbj, stat, upper);
-    out[15 * kWinField] = 1.0 + (double)stat;
-    const DPoint cur = dpoint(A, at(S.curr, F.N, i));
-    for (int n = 0; n < 4; ++n) {
-        double* o = out + (16 + 7 * n) * kWinField;
-        if ((stat >> n) & 1u) {
-            const LinForm L = lin_diff(dpoint(A, at(S.curr, F.N, nbj[n])), cur);
-            o[0] = L.k;
-            for (int j = 0; j < 6; ++j) o[(1 + j) * kWinField] = L.a[j];
-        } else {
-            for (int j = 0; j < 7; ++j) o[j * kWinField] = 0.0;
-        }
-    }
 }
 
-// The record of ray i in two parts for k_detect: the window form and width
-// (false for rays not searched, whose width field is -inf), which the
-// super-block's window bound needs, and the rest - the previous-time form,
-// the neighbour forms and the neighbour bits - read only by wavefronts some
-// of whose rays pass it.
+// The candidate region of ray i in two parts for k_detect: the window form
+// and width from the record (false for rays not searched, whose width field
+// is -inf), which the super-block's window bound needs, and the rest - the
+// previous-time form, the neighbour forms and the neighbours
+// is_closest_ray compares - read only by wavefronts some of whose rays pass
+// it.
 WQ_HD_INLINE bool load_cell_window(const FanConst& F, const Arrays& A, int64_t i, DetectCell& C) {
     if (i < 0 || i >= F.N) return false;
     const double* in = A.win + win_slot(F, i);
@@ -1173,48 +1158,43 @@ WQ_HD_INLINE bool load_cell_window(const FanConst& F, const Arrays& A, int64_t i
     return C.W.w != -HUGE_VAL;
 }
 
+// The rest of the region of a searched ray (detect_interior, as the record
+// was written for it): d2(prev) - d2(cur) and d2(neighbour) - d2(cur) from
+// the prev and curr slots' snapshots, and which neighbours are compared -
+// the AZ rule's (ray_neighbours) less the DE neighbours on an edge.  The
+// edge flags are the curr slot's, which k_advance wrote in the launch
+// before.  Rays not searched get no forms and no neighbours.
 WQ_HD_INLINE void load_cell_rest(const FanConst& F, const Arrays& A, const Slots& S, int64_t i, DetectCell& C) {
-    if (i < 0 || i >= F.N) return;
-    const double* in = A.win + win_slot(F, i);
-    const double searched = in[15 * kWinField];
-    C.W.prev.k = in[8 * kWinField];
-    for (int j = 0; j < 6; ++j) C.W.prev.a[j] = in[(9 + j) * kWinField];
-    for (int n = 0; n < 4; ++n) {
-        C.nb[n].k = in[(16 + 7 * n) * kWinField];
-        for (int j = 0; j < 6; ++j) C.nb[n].a[j] = in[(17 + 7 * n + j) * kWinField];
+    C.use = 0u;
+    if (!detect_interior(F, A, S, i)) {
+        C.W.prev.k = 0.0;
+        for (int j = 0; j < 6; ++j) C.W.prev.a[j] = 0.0;
+        return;
     }
-    const bool edge_lo = i - F.naz >= 0 && A.edge[at(S.curr, F.N, i - F.naz)] != 0;
-    const bool edge_hi = i + F.naz < F.N && A.edge[at(S.curr, F.N, i + F.naz)] != 0;
-    uint32_t use = searched != 0.0 ? (uint32_t)searched - 1u : 0u;
-    if ((use & 1u) && edge_lo) use &= ~1u;
-    if ((use & 2u) && edge_hi) use &= ~2u;
+    const int64_t N = F.N;
+    const DPoint cur = dpoint(A, at(S.curr, N, i));
+    C.W.prev = lin_diff(dpoint(A, at(S.prev, N, i)), cur);
+    int32_t nbj[4];
+    uint32_t stat, upper;
+    ray_neighbours(F, i, nbj, stat, upper);
+    for (int n = 0; n < 4; ++n) {
+        if ((stat >> n) & 1u) {
+            C.nb[n] = lin_diff(dpoint(A, at(S.curr, N, nbj[n])), cur);
+        } else {
+            C.nb[n].k = 0.0;
+            for (int j = 0; j < 6; ++j) C.nb[n].a[j] = 0.0;
+        }
+    }
+    uint32_t use = stat;
+    if ((use & 1u) && A.edge[at(S.curr, N, i - F.naz)] != 0) use &= ~1u;
+    if ((use & 2u) && A.edge[at(S.curr, N, i + F.naz)] != 0) use &= ~2u;
     C.use = use;
 }
 
-// the whole record of ray i as a DetectCell; false for rays not searched.
-// Every field and the DE neighbours' edge flags are read before the
-// searched flag is looked at, so the reads overlap in one round trip.
+// the whole region of ray i as a DetectCell; false for rays not searched
 WQ_HD_INLINE bool load_cell(const FanConst& F, const Arrays& A, const Slots& S, int64_t i, DetectCell& C) {
-    if (i < 0 || i >= F.N) return false;
-    const double* in = A.win + win_slot(F, i);
-    const double searched = in[15 * kWinField];
-    C.W.form.k = in[0];
-    for (int j = 0; j < 6; ++j) C.W.form.a[j] = in[(1 + j) * kWinField];
-    C.W.w = in[7 * kWinField];
-    C.W.prev.k = in[8 * kWinField];
-    for (int j = 0; j < 6; ++j) C.W.prev.a[j] = in[(9 + j) * kWinField];
-    C.W.m = F.lin_margin;
-    for (int n = 0; n < 4; ++n) {
-        C.nb[n].k = in[(16 + 7 * n) * kWinField];
-        for (int j = 0; j < 6; ++j) C.nb[n].a[j] = in[(17 + 7 * n + j) * kWinField];
-    }
-    const bool edge_lo = i - F.naz >= 0 && A.edge[at(S.curr, F.N, i - F.naz)] != 0;
-    const bool edge_hi = i + F.naz < F.N && A.edge[at(S.curr, F.N, i + F.naz)] != 0;
-    if (searched == 0.0) return false;
-    uint32_t use = (uint32_t)searched - 1u;
-    if ((use & 1u) && edge_lo) use &= ~1u;
-    if ((use & 2u) && edge_hi) use &= ~2u;
-    C.use = use;
+    if (!load_cell_window(F, A, i, C)) return false;
+    load_cell_rest(F, A, S, i, C);
     return true;
 }
 
