@@ -142,6 +142,47 @@ lib.wq3d_batch_destroy(h)
 
 
 @pytest.mark.gpu
+def test_two_batches_from_two_threads_gpu():
+    """Two batches stepped at once from two host threads, each building its
+    HIP graphs while the other steps and drains: every fan's records equal
+    its run alone."""
+    import threading
+    from underseamodelinglibrary_amd import _lib
+    from underseamodelinglibrary_amd.usml.waveq3d import fan_batch
+    lib = _lib.load()
+    cases = _cases()
+    print(f"[batch] two threads: {len(cases)} fans in two batches, {STEPS} steps", flush=True)
+    alone = []
+    for name, make, verbs in cases:
+        q = _queue(make(), lib, verbs)
+        lib.check(lib.wq3d_step(q._fan, STEPS))
+        alone.append(_records(q))
+        q.close()
+    queues = [_queue(make(), lib, verbs) for _, make, verbs in cases]
+    batches = [fan_batch(queues[:3], lib=lib), fan_batch(queues[3:], lib=lib)]
+    status = [None, None]
+
+    def run(k):
+        status[k] = lib.wq3d_batch_step(batches[k]._h, STEPS)
+
+    threads = [threading.Thread(target=run, args=(k,)) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert status == [0, 0], lib.wq3d_last_error().decode()
+    for (name, _, _), q, ref in zip(cases, queues, alone):
+        got = _records(q)
+        assert len(got) == len(ref)
+        for k, (x, y) in enumerate(zip(got, ref)):
+            assert x.tobytes() == y.tobytes(), (name, k)
+    for b in batches:
+        b.close()
+    for q in queues:
+        q.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["graph", "eager", "tuned"])
 def test_batch_equals_each_alone_gpu(mode):
     from underseamodelinglibrary_amd import _lib

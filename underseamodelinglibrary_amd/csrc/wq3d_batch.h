@@ -8,12 +8,14 @@
 // of one per fan: the fused launches (k_*_multi) give each fan a contiguous
 // range of blocks, so small fans fill the device together.  Every 64 steps
 // the record sets of all fans rotate, exactly as each fan's own wq3d_step
-// would rotate them; between rotations the 64 steps' launches are captured
-// once per record set in a HIP graph and replayed.  Launch shapes (threads
+// would rotate them; between rotations the 64 steps' launches are built
+// once per record set into a HIP graph and replayed.  Launch shapes (threads
 // per block of the per-ray launches, blocks of the construction launch) are
 // picked by wq3d_batch_tune, which times real steps of the batch: the shape
 // changes only how work is spread over the device, never a result.
 #include <hip/hip_runtime.h>
+
+#include <tuple>
 
 namespace {
 
@@ -31,6 +33,46 @@ hipError_t put(DeviceBuffers& mem, T** out, const T* src, size_t n) {
 bool identity_slots(const wq3d_fan* f) {
     return f->phys[0] == 0 && f->phys[1] == 1 && f->phys[2] == 2 && f->phys[3] == 3;
 }
+
+#ifndef WQ3D_EMULATE
+// The launches of a step go straight onto the batch's stream or, while a
+// graph is being built, into that graph as a chain of kernel nodes.  The
+// graphs are built node by node instead of by capturing the stream: while
+// a stream of the device is capturing, ROCm refuses legacy-stream work from
+// every other host thread (a record drain's copy, say) and the capture is
+// lost, so capturing would break batches stepped from several threads.
+struct Launcher {
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;     // set: add nodes instead of launching
+    hipGraphNode_t last = nullptr;  // the chain's last node
+    hipError_t err = hipSuccess;
+
+    template <class... P, class... A>
+    void operator()(void (*kernel)(P...), dim3 grid, dim3 block, A... args) {
+        if (err != hipSuccess) return;
+        if (!graph) {
+            hipLaunchKernelGGL(kernel, grid, block, 0, stream, args...);
+            return;
+        }
+        std::tuple<P...> vals(args...);
+        std::apply(
+            [&](auto&... v) {
+                void* ptrs[] = {static_cast<void*>(&v)...};
+                hipKernelNodeParams np{};
+                np.func = reinterpret_cast<void*>(kernel);
+                np.gridDim = grid;
+                np.blockDim = block;
+                np.sharedMemBytes = 0;
+                np.kernelParams = ptrs;
+                np.extra = nullptr;
+                hipGraphNode_t node = nullptr;
+                err = hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &np);
+                if (err == hipSuccess) last = node;
+            },
+            vals);
+    }
+};
+#endif
 
 }  // namespace
 
@@ -87,30 +129,28 @@ wq3d_status batch_tables(wq3d_batch* b) {
     return WQ3D_OK;
 }
 
-// one step's launches on the batch's stream for record set `rset`; the
-// host state (slots, times, record sets) is advanced by the caller
-wq3d_status batch_launch_step(wq3d_batch* b, const Slots& S, int rset) {
+// one step's launches for record set `rset`, onto the batch's stream or into
+// the graph being built (L); the host state (slots, times, record sets) is
+// advanced by the caller
+#ifndef WQ3D_EMULATE
+wq3d_status batch_launch_step(wq3d_batch* b, Launcher& L, const Slots& S, int rset) {
     const int n = (int)b->fans.size();
     const KParams* const* kps = b->kps + (size_t)rset * n;
-#ifndef WQ3D_EMULATE
-    const unsigned tpb = (unsigned)b->shape.ray_threads;
+    const dim3 tpb((unsigned)b->shape.ray_threads);
     for (const auto& g : b->groups)
-        hipLaunchKernelGGL(k_reflect_light_multi, dim3((unsigned)g.blocks), dim3(tpb), 0, b->stream, kps, g.d_ids,
-                           g.d_first, (int)g.ids.size(), S);
+        L(&k_reflect_light_multi, dim3((unsigned)g.blocks), tpb, kps, g.d_ids, g.d_first, (int)g.ids.size(), S);
     for (const auto& g : b->groups) {
         const int ng = (int)g.ids.size();
+        const dim3 grid((unsigned)g.blocks);
         switch (g.variant) {
         case 1:
-            hipLaunchKernelGGL((k_bounce_multi<WQ3D_PROFILE_MUNK, true>), dim3((unsigned)g.blocks), dim3(tpb), 0,
-                               b->stream, kps, g.d_ids, g.d_first, ng, S, b->clk);
+            L(&k_bounce_multi<WQ3D_PROFILE_MUNK, true>, grid, tpb, kps, g.d_ids, g.d_first, ng, S, b->clk);
             break;
         case 2:
-            hipLaunchKernelGGL((k_bounce_multi<WQ3D_PROFILE_LINEAR, true>), dim3((unsigned)g.blocks), dim3(tpb), 0,
-                               b->stream, kps, g.d_ids, g.d_first, ng, S, b->clk);
+            L(&k_bounce_multi<WQ3D_PROFILE_LINEAR, true>, grid, tpb, kps, g.d_ids, g.d_first, ng, S, b->clk);
             break;
         case 3:
-            hipLaunchKernelGGL((k_bounce_multi<kProfileGrid1Linear, true>), dim3((unsigned)g.blocks), dim3(tpb), 0,
-                               b->stream, kps, g.d_ids, g.d_first, ng, S, b->clk);
+            L(&k_bounce_multi<kProfileGrid1Linear, true>, grid, tpb, kps, g.d_ids, g.d_first, ng, S, b->clk);
             break;
         default:
             return fail(WQ3D_ERR_UNSUPPORTED, "environment kind without a batch variant");
@@ -120,25 +160,23 @@ wq3d_status batch_launch_step(wq3d_batch* b, const Slots& S, int rset) {
     const Slots R{S.prev, S.curr, S.next, S.past};
     for (const auto& g : b->groups) {
         const int ng = (int)g.ids.size();
+        const dim3 grid((unsigned)g.blocks);
         switch (g.variant) {
         case 1:
-            hipLaunchKernelGGL((k_advance_multi<WQ3D_PROFILE_MUNK, true>), dim3((unsigned)g.blocks), dim3(tpb), 0,
-                               b->stream, kps, g.d_ids, g.d_first, ng, R, b->clk);
+            L(&k_advance_multi<WQ3D_PROFILE_MUNK, true>, grid, tpb, kps, g.d_ids, g.d_first, ng, R, b->clk);
             break;
         case 2:
-            hipLaunchKernelGGL((k_advance_multi<WQ3D_PROFILE_LINEAR, true>), dim3((unsigned)g.blocks), dim3(tpb), 0,
-                               b->stream, kps, g.d_ids, g.d_first, ng, R, b->clk);
+            L(&k_advance_multi<WQ3D_PROFILE_LINEAR, true>, grid, tpb, kps, g.d_ids, g.d_first, ng, R, b->clk);
             break;
         default:
-            hipLaunchKernelGGL((k_advance_multi<kProfileGrid1Linear, true>), dim3((unsigned)g.blocks), dim3(tpb),
-                               0, b->stream, kps, g.d_ids, g.d_first, ng, R, b->clk);
+            L(&k_advance_multi<kProfileGrid1Linear, true>, grid, tpb, kps, g.d_ids, g.d_first, ng, R, b->clk);
             break;
         }
     }
     if (!b->tids.empty()) {
         const int nt = (int)b->tids.size();
         std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
-        if (b->timing) {
+        if (b->timing) {  // eager steps only (batch_run builds no graph while timing)
             if (!b->event_pool.empty()) {
                 ev = b->event_pool.back();
                 b->event_pool.pop_back();
@@ -148,22 +186,25 @@ wq3d_status batch_launch_step(wq3d_batch* b, const Slots& S, int rset) {
             }
             HIP_TRY(rt::event_record(ev.first, b->stream));
         }
-        hipLaunchKernelGGL(k_supmask_multi, dim3((unsigned)b->sup_blocks), dim3(kBlock), 0, b->stream, kps,
-                           b->d_tids, b->d_sup_first, nt);
-        hipLaunchKernelGGL(k_detect_multi, dim3((unsigned)b->det_blocks), dim3(kBlock), 0, b->stream, kps,
-                           b->d_tids, b->d_det_first, nt, R);
+        L(&k_supmask_multi, dim3((unsigned)b->sup_blocks), dim3(kBlock), kps, (const int32_t*)b->d_tids,
+          (const int64_t*)b->d_sup_first, nt);
+        L(&k_detect_multi, dim3((unsigned)b->det_blocks), dim3(kBlock), kps, (const int32_t*)b->d_tids,
+          (const int64_t*)b->d_det_first, nt, R);
         if (b->timing) {
             HIP_TRY(rt::event_record(ev.second, b->stream));
             b->stage_events.push_back(ev);
         }
-        hipLaunchKernelGGL(k_build_multi, dim3((unsigned)b->shape.build_blocks), dim3(kBlock), 0, b->stream, kps,
-                           b->d_tids, nt, R, b->clk, kBuildLanes);
+        L(&k_build_multi, dim3((unsigned)b->shape.build_blocks), dim3(kBlock), kps, (const int32_t*)b->d_tids, nt,
+          R, (const FanClock*)b->clk, kBuildLanes);
     }
-    HIP_TRY(rt::last_error());
+    if (L.err != hipSuccess) return fail(WQ3D_ERR_DEVICE, std::string("batch graph node: ") + hipGetErrorString(L.err));
+    if (!L.graph) HIP_TRY(rt::last_error());
+    return WQ3D_OK;
+}
 #else
-    // the emulation build steps the fans one after another through the
-    // single-fan launch paths, with the same slots, times and record sets
-    (void)kps;
+// the emulation build steps the fans one after another through the
+// single-fan launch paths, with the same slots, times and record sets
+wq3d_status batch_launch_step(wq3d_batch* b, const Slots& S, int rset) {
     const Slots R{S.prev, S.curr, S.next, S.past};
     for (wq3d_fan* f : b->fans) {
         const unsigned grid = (unsigned)((f->F.N + kBlock - 1) / kBlock);
@@ -179,9 +220,9 @@ wq3d_status batch_launch_step(wq3d_batch* b, const Slots& S, int rset) {
             launch_build(b->stream, f->kp + rset, f->env->dev, f->F, f->A, R, f->R, t, f->steps + 1);
         }
     }
-#endif
     return WQ3D_OK;
 }
+#endif
 
 // the host side of one step for every fan: the rotation, time and step
 // count as wq3d_step keeps them, and the record sets every 64 steps
@@ -207,7 +248,13 @@ wq3d_status batch_host_step(wq3d_batch* b) {
 
 wq3d_status batch_eager_step(wq3d_batch* b) {
     wq3d_fan* f0 = b->fans[0];
+#ifndef WQ3D_EMULATE
+    Launcher L;
+    L.stream = b->stream;
+    wq3d_status st = batch_launch_step(b, L, slots_of(f0), f0->rset);
+#else
     wq3d_status st = batch_launch_step(b, slots_of(f0), f0->rset);
+#endif
     if (st) return st;
 #ifdef WQ3D_EMULATE
     // the emulated launches read the host clocks directly
@@ -216,30 +263,31 @@ wq3d_status batch_eager_step(wq3d_batch* b) {
 }
 
 #ifndef WQ3D_EMULATE
-// 64 steps from the identity slots for record set `rset`, captured once
+// 64 steps from the identity slots for record set `rset`, built once
 wq3d_status batch_graph(wq3d_batch* b, int rset, hipGraphExec_t* out) {
     if (b->exec[rset]) {
         *out = b->exec[rset];
         return WQ3D_OK;
     }
-    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipGraphCreate(&graph, 0));
+    Launcher L;
+    L.stream = b->stream;
+    L.graph = graph;
     int phys[4] = {0, 1, 2, 3};
     wq3d_status st = WQ3D_OK;
     for (int k = 0; k < kBatchGraphSteps && !st; ++k) {
-        st = batch_launch_step(b, Slots{phys[0], phys[1], phys[2], phys[3]}, rset);
+        st = batch_launch_step(b, L, Slots{phys[0], phys[1], phys[2], phys[3]}, rset);
         const int save = phys[0];
         phys[0] = phys[1];
         phys[1] = phys[2];
         phys[2] = phys[3];
         phys[3] = save;
     }
-    hipGraph_t graph = nullptr;
-    const hipError_t e = hipStreamEndCapture(b->stream, &graph);
     if (st) {
-        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         return st;
     }
-    if (e != hipSuccess) return fail(WQ3D_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
     const hipError_t ei = hipGraphInstantiate(&b->exec[rset], graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ei != hipSuccess) return fail(WQ3D_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
