@@ -371,6 +371,35 @@ wq3d_status wq3d_set_timing(wq3d_fan* fan, int32_t enable);
 wq3d_status wq3d_kernel_times(wq3d_fan* fan, double* ms_per_launch, int64_t* launches,
                               int32_t num_kernels);
 
+/* ------------------------------------------------ adjoint (differentiation)
+ * The discrete adjoint of eigenray travel time and transmission loss (dB)
+ * with respect to the node speeds of the fan's sound speed grid.  No
+ * counterpart in the reference, which has no derivatives: it is for
+ * callers that fit or invert a sound speed profile (gradient descent over
+ * profiles, sensitivity kernels).  Supported: a 1-D profile grid with linear
+ * interpolation between flat boundaries, hybrid Gaussian spreading
+ * (WQ3D_ERR_UNSUPPORTED otherwise).
+ *
+ * wq3d_adjoint_checkpoint keeps a device copy of the fan's whole state; the
+ * first must be taken before the first step, later ones every K steps of
+ * the forward run (memory grows by one copy of the state per call).
+ * wq3d_adjoint_gradient takes eigenrays of this fan as drained
+ * (wq3d_drain_eigenrays: records, intensities [num_rays][num_freq]), the
+ * gradient of a loss with respect to each one's travel time and
+ * intensities, and writes d loss / d node speed into grad_nodes (num_nodes,
+ * the grid's size).  It replays each segment between checkpoints on a
+ * copy of the state (the fan itself is untouched and may step on) and
+ * checks that the replay rebuilds every given eigenray bit for bit
+ * (WQ3D_ERR_STATE if not: records of another fan or run).  The backward
+ * workspace holds a window of the segment's steps (416 bytes per ray and
+ * step) plus 1 GiB of tapes and entries; wq3d_adjoint_release frees it and
+ * the checkpoints. */
+wq3d_status wq3d_adjoint_checkpoint(wq3d_fan* fan);
+wq3d_status wq3d_adjoint_gradient(wq3d_fan* fan, const wq3d_eigenray* rays, const double* intensity,
+                                  const double* grad_time, const double* grad_intensity, int64_t num_rays,
+                                  double* grad_nodes, int64_t num_nodes);
+wq3d_status wq3d_adjoint_release(wq3d_fan* fan);
+
 /* Diagnostics with no counterpart in the reference: the eigenray search's
  * pair counts of the last step (pairs the first-stage filter kept - the
  * pre-test window and the neighbour forms - and pairs that passed the exact

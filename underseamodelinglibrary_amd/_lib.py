@@ -68,6 +68,9 @@ def bind(lib):
         "wq3d_batch_tune": ([vp, i32, dp, C.POINTER(i64)], i32),
         "wq3d_batch_set_timing": ([vp, i32], i32),
         "wq3d_batch_stage_times": ([vp, dp, C.POINTER(i64)], i32),
+        "wq3d_adjoint_checkpoint": ([vp], i32),
+        "wq3d_adjoint_gradient": ([vp, C.POINTER(_abi.EigenrayRec), dp, dp, dp, i64, dp, i64], i32),
+        "wq3d_adjoint_release": ([vp], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)

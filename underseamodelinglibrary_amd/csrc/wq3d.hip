@@ -575,6 +575,9 @@ namespace rt {
 inline hipError_t malloc(void** p, size_t n) { return hipMalloc(p, n); }
 inline hipError_t free(void* p) { return hipFree(p); }
 inline hipError_t memcpy(void* d, const void* s, size_t n, hipMemcpyKind k) { return hipMemcpy(d, s, n, k); }
+inline hipError_t memcpy_async(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t st) {
+    return hipMemcpyAsync(d, s, n, k, st);
+}
 inline hipError_t memset(void* d, int v, size_t n) { return hipMemset(d, v, n); }
 inline hipError_t memset_async(void* d, int v, size_t n, hipStream_t s) { return hipMemsetAsync(d, v, n, s); }
 inline hipError_t sync(hipStream_t s) { return hipStreamSynchronize(s); }
@@ -621,6 +624,9 @@ inline hipError_t free(void* p) {
 inline hipError_t memcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
     std::memcpy(d, s, n);
     return hipSuccess;
+}
+inline hipError_t memcpy_async(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) {
+    return memcpy(d, s, n, k);
 }
 inline hipError_t memset(void* d, int v, size_t n) {
     std::memset(d, v, n);
@@ -991,8 +997,10 @@ wq3d_status fail(wq3d_status s, const std::string& msg) {
 
 struct DeviceBuffers {
     std::vector<void*> ptrs;
-    ~DeviceBuffers() {
+    ~DeviceBuffers() { reset(); }
+    void reset() {
         for (void* p : ptrs) (void)rt::free(p);
+        ptrs.clear();
     }
     template <class T>
     hipError_t alloc(T** out, size_t count) {
@@ -1057,8 +1065,12 @@ struct wq3d_env {
     int device = 0;
 };
 
+struct FanAdjoint;  // wq3d_adjoint_host.h
+void adjoint_free(wq3d_fan* f);
+
 struct wq3d_fan {
     wq3d_env* env = nullptr;
+    FanAdjoint* adjoint = nullptr;  // checkpoints and backward workspace (wq3d_adjoint_*)
     KParams* kp = nullptr;   // device copies of {env, F, A, R}, one per record set
     bool kp_dirty = true;    // F changed since the last upload
     int device = 0;
@@ -1766,6 +1778,7 @@ wq3d_status wq3d_fan_destroy(wq3d_fan* f) {
     for (hipEvent_t e : f->event_pool) (void)rt::event_destroy(e);
     for (auto& r : f->rs)
         if (r.done) (void)rt::event_destroy(r.done);
+    adjoint_free(f);
     (void)rt::stream_destroy(f->stream);
     delete f;
     return WQ3D_OK;
@@ -2474,3 +2487,4 @@ int32_t wq3d_emu_check_detect(int64_t num_rays, int64_t num_targets, uint64_t se
 }  // extern "C"
 
 #include "wq3d_batch.h"
+#include "wq3d_adjoint_host.h"

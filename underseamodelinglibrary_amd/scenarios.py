@@ -208,21 +208,24 @@ def config5_ocean(speeds, bottom_depth=4500.0):
     return oc.ocean_model(oc.boundary_flat(), bottom, profile)
 
 
-def config5(profile=0, de=None, az=None, n_targets=(50, 50), time_max=120.0, freq=None, speeds=None):
+def config5(profile=0, de=None, az=None, n_targets=(50, 50), time_max=120.0, freq=None, speeds=None,
+            radius=6371000.0, half_width=0.5, source_depth=1000.0):
     """Config 5's forward run for one of its 32 profiles: 64-node piecewise
     linear c(z) (seeded Munk plus 2 m/s smoothed noise), range-independent,
     flat 4500 m bottom; DE -45:0.1:45 (901), AZ 0:5:355 (72), 500-4000 Hz
     (4 octave-spaced frequencies), dt 0.05 s, t < 120 s, 50 x 50 targets
-    within 0.5 deg of the source (45N 45W, 1000 m) at 100-2000 m depth."""
-    set_earth_radius(6371000.0)
+    within 0.5 deg of the source (45N 45W, 1000 m) at 100-2000 m depth.
+    `radius`, `half_width` (deg) and `source_depth` change the earth radius,
+    the target box and the source depth of derivative checks."""
+    set_earth_radius(radius)
     if speeds is None:
         speeds = config5_profile(profile)[1]
     ocean = config5_ocean(speeds)
     freq = freq if freq is not None else seq_log(500.0, 2.0, 4)
-    pos = wposition1(45.0, -45.0, -1000.0)
+    pos = wposition1(45.0, -45.0, -source_depth)
     de = de if de is not None else seq_linear(-45.0, 0.1, 45.0)
     az = az if az is not None else seq_linear(0.0, 5.0, 355.0)
-    tg = (target_grid(45.0, -45.0, 0.5, n_targets[0], n_targets[1], 100.0, 2000.0)
+    tg = (target_grid(45.0, -45.0, half_width, n_targets[0], n_targets[1], 100.0, 2000.0)
           if n_targets else None)
     sc = scenario("config5", ocean, freq, pos, de, az, 0.05, tg, loop_steps(0.05, time_max))
     sc.notes["profile"] = profile

@@ -505,6 +505,32 @@ class wave_queue(queue_base):
             fs = refls[refls["step"] == s - 1] if refls.size else refls
             self._deliver(self._time, self._eigenray_models(rs, ri, rp), vm, fs)
 
+    # -- differentiation (no counterpart in the reference; include/wq3d.h)
+    def adjoint_checkpoint(self):
+        """Keep a device copy of the whole fan state for the backward pass:
+        once before the first step, then every K steps of the run."""
+        self._lib.check(self._lib.wq3d_adjoint_checkpoint(self._fan))
+
+    def adjoint_gradient(self, recs, intensity, grad_time, grad_intensity, num_nodes):
+        """d loss / d node speed of the sound speed grid, for eigenray records
+        of this fan (as drain_eigenrays returns them) and the loss gradients
+        with respect to their travel times [n] and intensities [n, nf] (dB)."""
+        nf = self._freq.size()
+        n = len(recs)
+        recs = np.ascontiguousarray(recs)
+        inten = np.ascontiguousarray(intensity, dtype=np.float64).reshape(n * nf)
+        gt = np.ascontiguousarray(grad_time, dtype=np.float64).reshape(n)
+        gi = np.ascontiguousarray(grad_intensity, dtype=np.float64).reshape(n * nf)
+        out = np.zeros(int(num_nodes))
+        dp = C.POINTER(C.c_double)
+        self._lib.check(self._lib.wq3d_adjoint_gradient(
+            self._fan, recs.ctypes.data_as(C.POINTER(_abi.EigenrayRec)), inten.ctypes.data_as(dp),
+            gt.ctypes.data_as(dp), gi.ctypes.data_as(dp), n, out.ctypes.data_as(dp), int(num_nodes)))
+        return out
+
+    def adjoint_release(self):
+        self._lib.check(self._lib.wq3d_adjoint_release(self._fan))
+
     # -- raw record access (numpy structured arrays), used by tests and bench
     def drain_eigenrays(self):
         lib = self._lib
