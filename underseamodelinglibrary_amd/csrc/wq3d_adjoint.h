@@ -45,7 +45,7 @@ enum Field {
     kSr, kSt, kSp, kSs,  // target-distance snapshot (tr, tt, tp, ts)
     kAtt                 // + frequency
 };
-constexpr int kHistFields = 13;  // history: kPr .. kC per slot
+constexpr int kStateFields = 13;  // kPr .. kC: what update() and a reflection write
 
 WQ_HD_INLINE int32_t code(int field, int slot) { return field * 4 + slot; }
 
@@ -1108,17 +1108,26 @@ WQ_HD_INLINE void step_vjp(const DEnv& E, const FanConst& F, const StepIn& in, c
 
 
 // ------------------------------------------------------------ backward-pass bodies
-// Device state of a fan's backward pass.
+// Device state of a fan's backward pass.  The history keeps, for each step
+// of the segment being differentiated, the fan's four slots as they were
+// after the step: kPr .. kSs of every slot ([step][kAtt][4][N]), the
+// attenuations in the fan's own layout ([step][4][nf][N]), the counters
+// ([step][5][4][N]: surface, bottom, caustic, upper, lower), edge flags
+// ([step][4][N]) and flag[] ([step][N]).
 struct AdjDev {
     double* adj;              // [kAtt + nf][4][N]: adjoint of every slot field
     double* node_grad;        // [nodes]
-    const double* hist;       // [calls][kHistFields][4][N]: post-step windows of the segment
-    const uint8_t* hflag;     // [calls][N]: flag[] after each of those steps
+    const double* hist;
+    const double* hatt;
+    const int32_t* hint;
+    const uint8_t* hedge;
+    const uint8_t* hflag;
     int64_t N;
+    int32_t nf;
     ad::Node* tape_node;      // tapes of tape_cap operations each
     double* tape_adj;
     int32_t tape_cap;
-    int64_t* ent_idx;         // seed entries: adjoint index, value
+    unsigned long long* ent_key;  // seed entries: (ray << 24) | (step << 8) | field code
     double* ent_val;
     unsigned long long* ent_count;
     int64_t ent_cap;
@@ -1126,19 +1135,42 @@ struct AdjDev {
                               // 3 eigenray not rebuilt, 4 reflection chain too long
 };
 
-// the eigenrays the caller differentiates, one step's worth at a time
+// the eigenrays the caller differentiates, with their step in the segment
 struct AdjSeed {
     int64_t target;
     int64_t ray;
     double travel_time;       // the forward run's value, checked by the replay
     double grad_time;         // dL / d travel_time
+    int64_t h;                // step of the segment (1 .. segment length)
 };
+
+constexpr int kEntryStepBits = 16;  // segments of at most 65535 steps
 
 WQ_HD_INLINE double* adj_at(const AdjDev& D, int field, int slot, int64_t i) {
     return D.adj + (int64_t)code(field, slot) * D.N + i;
 }
 WQ_HD_INLINE double hist_at(const AdjDev& D, int64_t h, int field, int slot, int64_t i) {
-    return D.hist[((h * kHistFields + field) * 4 + slot) * D.N + i];
+    return D.hist[((h * kAtt + field) * 4 + slot) * D.N + i];
+}
+
+// the fan's arrays as they were after step h of the segment (read only)
+WQ_HD_INLINE Arrays hist_view(const AdjDev& D, int64_t h) {
+    const int64_t n4 = 4 * D.N;
+    double* b = const_cast<double*>(D.hist) + h * kAtt * n4;
+    Arrays A{};
+    double** f[kAtt] = {&A.pr, &A.pt, &A.pp, &A.nr, &A.nt, &A.np, &A.gr, &A.gt, &A.gp,
+                        &A.hr, &A.ht, &A.hp, &A.c, &A.tr, &A.tt, &A.tp, &A.ts};
+    for (int k = 0; k < kAtt; ++k) *f[k] = b + k * n4;
+    A.atten = const_cast<double*>(D.hatt) + h * n4 * D.nf;
+    int32_t* ib = const_cast<int32_t*>(D.hint) + h * 5 * n4;
+    A.srf = ib;
+    A.btm = ib + n4;
+    A.cst = ib + 2 * n4;
+    A.upr = ib + 3 * n4;
+    A.lwr = ib + 4 * n4;
+    A.edge = const_cast<uint8_t*>(D.hedge) + h * n4;
+    A.flag = const_cast<uint8_t*>(D.hflag) + h * D.N;
+    return A;
 }
 
 WQ_HD_INLINE void atom_add_d(double* p, double v) {
@@ -1159,30 +1191,40 @@ WQ_HD_INLINE void flush_nodes(const AdjDev& D, const ad::Tape& t) {
         if (t.node[k].a == ad::kInputNode && t.adj[k] != 0.0) atom_add_d(D.node_grad + t.node[k].b, t.adj[k]);
 }
 
-// window capture: the post-step state the adjoint of the step needs
-WQ_HD_INLINE void body_adj_capture(int64_t i, const FanConst& F, const Arrays& A, double* hist, uint8_t* hflag,
-                                   int64_t h) {
-    const double* src[kHistFields] = {A.pr, A.pt, A.pp, A.nr, A.nt, A.np, A.gr, A.gt, A.gp, A.hr, A.ht, A.hp, A.c};
+// history capture of ray i after step h of the segment
+WQ_HD_INLINE void body_adj_capture(int64_t i, const FanConst& F, const Arrays& A, const AdjDev& D, int64_t h) {
+    const Arrays H = hist_view(D, h);
     const int64_t N = F.N;
-    for (int f = 0; f < kHistFields; ++f)
-        for (int s = 0; s < 4; ++s) hist[((h * kHistFields + f) * 4 + s) * N + i] = src[f][s * N + i];
-    hflag[h * N + i] = A.flag[i];
+    const double* src[kAtt] = {A.pr, A.pt, A.pp, A.nr, A.nt, A.np, A.gr, A.gt, A.gp,
+                               A.hr, A.ht, A.hp, A.c, A.tr, A.tt, A.tp, A.ts};
+    double* dst[kAtt] = {H.pr, H.pt, H.pp, H.nr, H.nt, H.np, H.gr, H.gt, H.gp,
+                         H.hr, H.ht, H.hp, H.c, H.tr, H.tt, H.tp, H.ts};
+    for (int f = 0; f < kAtt; ++f)
+        for (int s = 0; s < 4; ++s) dst[f][s * N + i] = src[f][s * N + i];
+    for (int k = 0; k < 4 * F.nf; ++k) H.atten[k * N + i] = A.atten[k * N + i];
+    const int32_t* is[5] = {A.srf, A.btm, A.cst, A.upr, A.lwr};
+    int32_t* id[5] = {H.srf, H.btm, H.cst, H.upr, H.lwr};
+    for (int f = 0; f < 5; ++f)
+        for (int s = 0; s < 4; ++s) id[f][s * N + i] = is[f][s * N + i];
+    for (int s = 0; s < 4; ++s) H.edge[s * N + i] = A.edge[s * N + i];
+    H.flag[i] = A.flag[i];
 }
 
-// Seeds of one eigenray of the step: its construction replayed on a tape
-// from the segment's replayed fan state, the caller's gradients pushed back
-// to what it read; the adjoints of state fields become entries applied at
-// the step's turn in the backward sweep, the node speeds' go straight to
-// node_grad.
-WQ_HD void body_adj_seed(int64_t e, int64_t tape_k, const DEnv& E, const FanConst& F, const Arrays& A,
-                         const Slots& S, double time, const AdjSeed* seeds, const double* seed_int,
+// Seeds of one eigenray: its construction replayed on a tape from the
+// history of its step, the caller's gradients pushed back to what it read;
+// the adjoints of state fields become entries keyed by (ray, step, field),
+// applied at the step's turn in the backward sweep; the node speeds' go
+// straight to node_grad.
+WQ_HD void body_adj_seed(int64_t e, int64_t tape_k, const DEnv& E, const FanConst& F, const Slots* slots,
+                         const double* times, const AdjSeed* seeds, const double* seed_int,
                          const double* seed_grad, const AdjDev& D) {
     ad::Tape tp;
     tape_of(D, tape_k, tp);
     const AdjSeed sd = seeds[e];
+    const Arrays A = hist_view(D, sd.h);
     V T;
     V I[kMaxFreq];
-    if (!eigenray_v(E, F, A, S, &tp, sd.target, sd.ray, time, T, I)) {
+    if (!eigenray_v(E, F, A, slots[sd.h], &tp, sd.target, sd.ray, times[sd.h], T, I)) {
         atom_or(D.err, 8);
         return;
     }
@@ -1209,7 +1251,8 @@ WQ_HD void body_adj_seed(int64_t e, int64_t tape_k, const DEnv& E, const FanCons
     for (int32_t k = 0; k < tp.n; ++k) {
         const ad::Node& nd = tp.node[k];
         if (nd.a != ad::kInputState || tp.adj[k] == 0.0) continue;
-        D.ent_idx[w] = (int64_t)nd.b * D.N + (int64_t)nd.da;
+        D.ent_key[w] = ((unsigned long long)nd.da << (8 + kEntryStepBits)) |
+                       ((unsigned long long)sd.h << 8) | (unsigned long long)nd.b;
         D.ent_val[w] = tp.adj[k];
         ++w;
     }
@@ -1314,9 +1357,9 @@ WQ_HD void body_adj_reflect(int64_t i, int64_t tape_k, const DEnv& E, const FanC
         bool same = true;
         for (int q = 1; q < 4; ++q) {
             const RayT<V>& r = w.s[q];
-            const V* f13[kHistFields] = {&r.pr, &r.pt, &r.pp, &r.nr, &r.nt, &r.np, &r.gr,
+            const V* f13[kStateFields] = {&r.pr, &r.pt, &r.pp, &r.nr, &r.nt, &r.np, &r.gr,
                                          &r.gt, &r.gp, &r.hr, &r.ht, &r.hp, &r.c};
-            for (int fld = 0; fld < kHistFields; ++fld)
+            for (int fld = 0; fld < kStateFields; ++fld)
                 same = same && f13[fld]->v == hist_at(D, h_post, fld, sq[q - 1], i);
         }
         if (!same) atom_or(D.err, 2);
@@ -1324,9 +1367,9 @@ WQ_HD void body_adj_reflect(int64_t i, int64_t tape_k, const DEnv& E, const FanC
     ad::clear_adjoints(&tp);
     for (int q = 0; q < 4; ++q) {
         const RayT<V>& r = w.s[q];
-        const V* f13[kHistFields] = {&r.pr, &r.pt, &r.pp, &r.nr, &r.nt, &r.np, &r.gr,
+        const V* f13[kStateFields] = {&r.pr, &r.pt, &r.pp, &r.nr, &r.nt, &r.np, &r.gr,
                                      &r.gt, &r.gp, &r.hr, &r.ht, &r.hp, &r.c};
-        for (int fld = 0; fld < kHistFields; ++fld) {
+        for (int fld = 0; fld < kStateFields; ++fld) {
             double* a = adj_at(D, fld, sp[q], i);
             ad::seed(&tp, *f13[fld], *a);
             *a = 0.0;
@@ -1343,6 +1386,32 @@ WQ_HD void body_adj_reflect(int64_t i, int64_t tape_k, const DEnv& E, const FanC
         if (nd.a == ad::kInputState && tp.adj[k] != 0.0) D.adj[(int64_t)nd.b * D.N + (int64_t)nd.da] += tp.adj[k];
     }
     flush_nodes(D, tp);
+}
+
+// The backward sweep of ray i over the segment's steps K .. 1: the step's
+// seed entries for the ray (sorted by ray, then step: the ray's range
+// [beg, end) is read from its end), the adjoint of its advance, the
+// adjoint of its reflections.  Rays are independent here - the seeds carry
+// every dependence between rays - so one thread takes a ray through the
+// whole segment.  The advance's node adjoints go to add_node(k, d0, d1).
+template <class AddNode>
+WQ_HD void body_adj_sweep(int64_t i, int64_t tape_k, const DEnv& E, const FanConst& F, const Slots* slots,
+                          int64_t K, const unsigned long long* key, const double* val, int64_t beg, int64_t end,
+                          const AdjDev& D, AddNode add_node) {
+    int64_t p = end;
+    for (int64_t h = K; h >= 1; --h) {
+        while (p > beg) {
+            const unsigned long long k = key[p - 1];
+            if ((int64_t)((k >> 8) & ((1ull << kEntryStepBits) - 1)) != h) break;
+            D.adj[(int64_t)(k & 0xffull) * D.N + i] += val[p - 1];
+            --p;
+        }
+        int k0;
+        double d0, d1;
+        body_adj_advance(i, E, F, slots[h], h, D, k0, d0, d1);
+        add_node(k0, d0, d1);
+        if (D.hflag[(h - 1) * D.N + i] != 0) body_adj_reflect(i, tape_k, E, F, slots[h - 1], slots[h], h - 1, h, D);
+    }
 }
 
 // The adjoint of the fan's start (body_init): the node speeds' share of
@@ -1397,9 +1466,9 @@ WQ_HD void body_adj_init(int64_t i, int64_t tape_k, const DEnv& E, const FanCons
     ad::clear_adjoints(&tp);
     for (int s = 0; s < 4; ++s) {
         const RayT<V>& r = *st[s];
-        const V* f13[kHistFields] = {&r.pr, &r.pt, &r.pp, &r.nr, &r.nt, &r.np, &r.gr,
+        const V* f13[kStateFields] = {&r.pr, &r.pt, &r.pp, &r.nr, &r.nt, &r.np, &r.gr,
                                      &r.gt, &r.gp, &r.hr, &r.ht, &r.hp, &r.c};
-        for (int fld = 0; fld < kHistFields; ++fld) ad::seed(&tp, *f13[fld], *adj_at(D, fld, sl[s], i));
+        for (int fld = 0; fld < kStateFields; ++fld) ad::seed(&tp, *f13[fld], *adj_at(D, fld, sl[s], i));
         // the snapshot of update(): position and sin(theta)
         ad::seed(&tp, r.pr, *adj_at(D, kSr, sl[s], i));
         ad::seed(&tp, r.pt, *adj_at(D, kSt, sl[s], i));

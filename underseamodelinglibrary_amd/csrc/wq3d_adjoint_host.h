@@ -4,13 +4,22 @@
 // A fan being differentiated keeps a copy of its whole state every time the
 // caller asks (wq3d_adjoint_checkpoint, every K steps of the forward run).
 // wq3d_adjoint_gradient then walks the segments between checkpoints from
-// the last to the first: it restores the segment's checkpoint into a
-// replay copy of the fan's arrays, steps it forward with the ordinary
-// reflection and advance kernels (no eigenray search: the caller's
-// eigenrays say where the seeds are), keeping every step's window and
-// computing the seeds of that step's eigenrays, and then runs the step
-// adjoints of the segment in reverse order (wq3d_adjoint.h).  The fan's own
-// state is not touched, so it may go on stepping afterwards.
+// the last to the first.  For each segment it
+//   1. restores the checkpoint into a replay copy of the fan's arrays and
+//      steps it forward with the ordinary reflection and advance kernels (no
+//      eigenray search: the caller's eigenrays say where the seeds are),
+//      keeping every step's four slots in a history;
+//   2. builds the seeds of all the segment's eigenrays at once, each from
+//      its step's history, as entries (ray, step, field, value);
+//   3. sorts the entries by ray and step (radix sort);
+//   4. sweeps every ray backward through the segment in one launch, one
+//      thread per ray: entries, step adjoint, reflection adjoint
+//      (wq3d_adjoint.h) - the rays are independent once the seeds exist.
+// The fan's own state is not touched, so it may go on stepping afterwards.
+#ifndef WQ3D_EMULATE
+#include <hipcub/hipcub.hpp>
+#endif
+
 #include "wq3d_adjoint.h"
 
 namespace {
@@ -18,16 +27,16 @@ namespace {
 using wq3d::adj::AdjDev;
 using wq3d::adj::AdjSeed;
 
-constexpr int32_t kTapeSmall = 4096;    // operations of a reflection or start-up tape
-constexpr int32_t kTapeSeed = 65536;    // operations of an eigenray's tape
+constexpr int32_t kTapeSmall = 2048;    // operations of a reflection (chain) or start-up tape
+constexpr int32_t kTapeSeed = 16384;    // operations of an eigenray's tape
 constexpr int kAdjBlock = 256;
 
 #ifndef WQ3D_EMULATE
-constexpr size_t kTapePoolBytes = size_t(512) << 20;
-constexpr int64_t kEntriesPerSegment = int64_t(1) << 24;
+constexpr size_t kTapePoolBytes = size_t(2) << 30;
+constexpr int64_t kEntriesPerSegment = int64_t(1) << 25;
 #else
 constexpr size_t kTapePoolBytes = size_t(16) << 20;
-constexpr int64_t kEntriesPerSegment = int64_t(1) << 20;
+constexpr int64_t kEntriesPerSegment = int64_t(1) << 22;
 #endif
 
 struct AdjCheckpoint {
@@ -50,9 +59,13 @@ struct FanAdjoint {
     KParams* kpw = nullptr;     // {E, F without records, W, Rw}
     Records Rw{};
     AdjDev D{};
-    double* hist = nullptr;
-    uint8_t* hflag = nullptr;
-    unsigned long long* eoff = nullptr;  // [hist_calls] entry counts after each replayed step
+    Slots* slots = nullptr;     // [hist_calls] slots after each replayed step
+    double* times = nullptr;    // [hist_calls] time after each replayed step
+    unsigned long long *skey = nullptr;  // sorted entries
+    double* sval = nullptr;
+    int64_t *rbeg = nullptr, *rend = nullptr;  // [N] each ray's range of sorted entries
+    void* sort_tmp = nullptr;
+    size_t sort_bytes = 0;
     AdjSeed* seeds = nullptr;
     double *seed_int = nullptr, *seed_grad = nullptr;
     int64_t seed_cap = 0;
@@ -62,59 +75,56 @@ struct FanAdjoint {
 
 namespace {
 
+int64_t tapes_of(int32_t cap) {
+    return (int64_t)(kTapePoolBytes / (sizeof(wq3d::ad::Node) + sizeof(double)) / (size_t)cap) & ~int64_t(63);
+}
+
 #ifndef WQ3D_EMULATE
-__global__ void __launch_bounds__(kAdjBlock) k_adj_capture(FanConst F, Arrays A, double* hist, uint8_t* hflag,
-                                                           int64_t h) {
+__global__ void __launch_bounds__(kAdjBlock) k_adj_capture(FanConst F, Arrays A, AdjDev D, int64_t h) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < F.N) wq3d::adj::body_adj_capture(i, F, A, hist, hflag, h);
+    if (i < F.N) wq3d::adj::body_adj_capture(i, F, A, D, h);
 }
 
-__global__ void k_adj_mark(const unsigned long long* count, unsigned long long* eoff, int64_t h) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) eoff[h] = *count;
-}
-
-// the seed entries of replayed step h, added to the adjoint state
-__global__ void __launch_bounds__(kAdjBlock) k_adj_scatter(AdjDev D, const unsigned long long* eoff, int64_t h) {
-    const unsigned long long b = eoff[h - 1], e = eoff[h];
-    for (unsigned long long k = b + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; k < e;
-         k += (unsigned long long)gridDim.x * blockDim.x)
-        atomicAdd(D.adj + D.ent_idx[k], D.ent_val[k]);
-}
-
-__global__ void __launch_bounds__(64) k_adj_seed(const KParams* __restrict__ P, Slots S, double time,
+__global__ void __launch_bounds__(64) k_adj_seed(DEnv E, FanConst F, const Slots* slots, const double* times,
                                                  const AdjSeed* seeds, const double* seed_int,
-                                                 const double* seed_grad, int64_t lo, int64_t hi, AdjDev D) {
+                                                 const double* seed_grad, int64_t n, AdjDev D) {
     const int64_t tk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nth = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t e = lo + tk; e < hi; e += nth)
-        wq3d::adj::body_adj_seed(e, tk, P->E, P->F, P->A, S, time, seeds, seed_int, seed_grad, D);
+    for (int64_t e = tk; e < n; e += nth)
+        wq3d::adj::body_adj_seed(e, tk, E, F, slots, times, seeds, seed_int, seed_grad, D);
 }
 
-// one ray per thread; the node adjoints are summed per block in LDS first
-__global__ void __launch_bounds__(kAdjBlock) k_adj_advance(DEnv E, FanConst F, Slots S, int64_t h, AdjDev D,
-                                                           int32_t n_nodes) {
+// each ray's range in the sorted entries
+__global__ void __launch_bounds__(kAdjBlock) k_adj_ranges(const unsigned long long* key, int64_t n, int64_t* beg,
+                                                          int64_t* end) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int shift = 8 + wq3d::adj::kEntryStepBits;
+    const int64_t r = (int64_t)(key[k] >> shift);
+    if (k == 0 || (int64_t)(key[k - 1] >> shift) != r) beg[r] = k;
+    if (k == n - 1 || (int64_t)(key[k + 1] >> shift) != r) end[r] = k + 1;
+}
+
+// one ray per thread through the segment; the node adjoints are summed per
+// block in LDS first
+__global__ void __launch_bounds__(kAdjBlock) k_adj_sweep(DEnv E, FanConst F, const Slots* slots, int64_t K,
+                                                         const unsigned long long* key, const double* val,
+                                                         const int64_t* beg, const int64_t* end, AdjDev D,
+                                                         int32_t n_nodes) {
     extern __shared__ double s_node[];
     for (int k = threadIdx.x; k < n_nodes; k += blockDim.x) s_node[k] = 0.0;
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < F.N) {
-        int k0;
-        double d0, d1;
-        wq3d::adj::body_adj_advance(i, E, F, S, h, D, k0, d0, d1);
+    const int64_t tk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
+    auto add = [&](int k0, double d0, double d1) {
         if (d0 != 0.0) atomicAdd(&s_node[k0], d0);
         if (d1 != 0.0) atomicAdd(&s_node[k0 + 1], d1);
-    }
+    };
+    for (int64_t i = tk; i < F.N; i += nth)
+        wq3d::adj::body_adj_sweep(i, tk, E, F, slots, K, key, val, beg[i], end[i], D, add);
     __syncthreads();
     for (int k = threadIdx.x; k < n_nodes; k += blockDim.x)
         if (s_node[k] != 0.0) atomicAdd(D.node_grad + k, s_node[k]);
-}
-
-__global__ void __launch_bounds__(64) k_adj_reflect(DEnv E, FanConst F, Slots Sp, Slots Sq, int64_t hp,
-                                                    int64_t hq, AdjDev D) {
-    const int64_t tk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = tk; i < F.N; i += nth)
-        if (D.hflag[hp * F.N + i] != 0) wq3d::adj::body_adj_reflect(i, tk, E, F, Sp, Sq, hp, hq, D);
 }
 
 __global__ void __launch_bounds__(64) k_adj_init(DEnv E, FanConst F, Slots S, AdjDev D) {
@@ -180,13 +190,17 @@ wq3d_status adjoint_workspace(wq3d_fan* f, int64_t calls) {
     if (copy_state(m, F, a.W, f->A, f->stream, true)) return WQ3D_ERR_DEVICE;
     a.W.win = f->A.win;
     a.W.tsup = f->A.tsup;
+    auto nomem = [&](const char* what) {
+        return fail(WQ3D_ERR_DEVICE, std::string("adjoint workspace allocation failed (") + what + ", history of " +
+                                         std::to_string(calls) + " steps)");
+    };
     {
         unsigned long long* c = nullptr;
         int32_t* e = nullptr;
         if (m.alloc(&c, 6 + 2 * kScanSlots) != hipSuccess || m.alloc(&e, 1) != hipSuccess ||
             rt::memset(c, 0, (6 + 2 * kScanSlots) * sizeof(unsigned long long)) != hipSuccess ||
             rt::memset(e, 0, sizeof(int32_t)) != hipSuccess)
-            return fail(WQ3D_ERR_DEVICE, "adjoint workspace allocation failed");
+            return nomem("records");
         Records& R = a.Rw;
         R = Records{};
         R.ray_count = c;
@@ -202,7 +216,7 @@ wq3d_status adjoint_workspace(wq3d_fan* f, int64_t calls) {
     FanConst Fw = F;
     Fw.want_verbs = 0;
     Fw.want_refl = 0;
-    if (m.alloc(&a.kpw, 1) != hipSuccess) return fail(WQ3D_ERR_DEVICE, "adjoint workspace allocation failed");
+    if (m.alloc(&a.kpw, 1) != hipSuccess) return nomem("parameters");
     {
         const KParams kp{f->env->dev, Fw, a.W, a.Rw};
         if (rt::memcpy(a.kpw, &kp, sizeof(KParams), hipMemcpyHostToDevice) != hipSuccess)
@@ -211,28 +225,49 @@ wq3d_status adjoint_workspace(wq3d_fan* f, int64_t calls) {
     AdjDev& D = a.D;
     D = AdjDev{};
     D.N = N;
-    const int64_t nadj = (int64_t)(wq3d::adj::kAtt + F.nf) * 4 * N;
+    D.nf = F.nf;
+    const int64_t n4 = 4 * N;
+    const int64_t nadj = (int64_t)(wq3d::adj::kAtt + F.nf) * n4;
     const size_t tape_nodes = kTapePoolBytes / (sizeof(wq3d::ad::Node) + sizeof(double));
-    if (m.alloc(&D.adj, nadj) != hipSuccess || m.alloc(&D.node_grad, std::max<int64_t>(1, a.n_nodes)) != hipSuccess ||
-        m.alloc(&a.hist, (size_t)calls * wq3d::adj::kHistFields * 4 * N) != hipSuccess ||
-        m.alloc(&a.hflag, (size_t)calls * N) != hipSuccess || m.alloc(&a.eoff, calls) != hipSuccess ||
-        m.alloc(&D.tape_node, tape_nodes) != hipSuccess || m.alloc(&D.tape_adj, tape_nodes) != hipSuccess ||
-        m.alloc(&D.ent_idx, kEntriesPerSegment) != hipSuccess || m.alloc(&D.ent_val, kEntriesPerSegment) != hipSuccess ||
+    double *hist = nullptr, *hatt = nullptr;
+    int32_t* hint = nullptr;
+    uint8_t *hedge = nullptr, *hflag = nullptr;
+    if (m.alloc(&D.adj, nadj) != hipSuccess || m.alloc(&D.node_grad, std::max<int64_t>(1, a.n_nodes)) != hipSuccess)
+        return nomem("adjoint state");
+    if (m.alloc(&hist, (size_t)calls * wq3d::adj::kAtt * n4) != hipSuccess ||
+        m.alloc(&hatt, (size_t)calls * F.nf * n4) != hipSuccess || m.alloc(&hint, (size_t)calls * 5 * n4) != hipSuccess ||
+        m.alloc(&hedge, (size_t)calls * n4) != hipSuccess || m.alloc(&hflag, (size_t)calls * N) != hipSuccess)
+        return nomem("history");
+    if (m.alloc(&D.tape_node, tape_nodes) != hipSuccess || m.alloc(&D.tape_adj, tape_nodes) != hipSuccess)
+        return nomem("tapes");
+    if (m.alloc(&D.ent_key, kEntriesPerSegment) != hipSuccess || m.alloc(&D.ent_val, kEntriesPerSegment) != hipSuccess ||
+        m.alloc(&a.skey, kEntriesPerSegment) != hipSuccess || m.alloc(&a.sval, kEntriesPerSegment) != hipSuccess ||
+        m.alloc(&a.rbeg, N) != hipSuccess || m.alloc(&a.rend, N) != hipSuccess ||
         m.alloc(&D.ent_count, 1) != hipSuccess || m.alloc(&D.err, 1) != hipSuccess)
-        return fail(WQ3D_ERR_DEVICE, "adjoint workspace allocation failed (history of " + std::to_string(calls) +
-                                         " steps)");
-    D.hist = a.hist;
-    D.hflag = a.hflag;
+        return nomem("seed entries");
+    if (m.alloc(&a.slots, calls) != hipSuccess || m.alloc(&a.times, calls) != hipSuccess) return nomem("steps");
+#ifndef WQ3D_EMULATE
+    a.sort_bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, a.sort_bytes, D.ent_key, a.skey, D.ent_val, a.sval,
+                                           (int)kEntriesPerSegment, 0, 64, f->stream) != hipSuccess)
+        return fail(WQ3D_ERR_DEVICE, "radix sort set-up failed");
+    uint8_t* tmp = nullptr;
+    if (m.alloc(&tmp, a.sort_bytes) != hipSuccess) return nomem("sort");
+    a.sort_tmp = tmp;
+#endif
+    D.hist = hist;
+    D.hatt = hatt;
+    D.hint = hint;
+    D.hedge = hedge;
+    D.hflag = hflag;
     D.ent_cap = kEntriesPerSegment;
     a.hist_calls = calls;
     a.have_ws = true;
     return WQ3D_OK;
 }
 
-// one replayed step of the fan's replay arrays (the reflections and the
-// advance of wq3d_step, no search)
+// the replayed fan's slot order, time and step count
 struct Replay {
-    wq3d_fan* f;
     int phys[4];
     double time;
     int64_t steps;
@@ -244,13 +279,6 @@ struct Replay {
         phys[2] = phys[3];
         phys[3] = save;
     }
-    void unrotate() {
-        const int save = phys[3];
-        phys[3] = phys[2];
-        phys[2] = phys[1];
-        phys[1] = phys[0];
-        phys[0] = save;
-    }
 };
 
 void launch_adj_capture(wq3d_fan* f, int64_t h) {
@@ -258,87 +286,83 @@ void launch_adj_capture(wq3d_fan* f, int64_t h) {
     const FanConst& F = f->F;
 #ifndef WQ3D_EMULATE
     const unsigned grid = (unsigned)((F.N + kAdjBlock - 1) / kAdjBlock);
-    hipLaunchKernelGGL(k_adj_capture, dim3(grid), dim3(kAdjBlock), 0, f->stream, F, a.W, a.hist, a.hflag, h);
+    hipLaunchKernelGGL(k_adj_capture, dim3(grid), dim3(kAdjBlock), 0, f->stream, F, a.W, a.D, h);
 #else
     emu_check_arrays(F, a.W);
-    for (int64_t i = 0; i < F.N; ++i) wq3d::adj::body_adj_capture(i, F, a.W, a.hist, a.hflag, h);
+    for (int64_t i = 0; i < F.N; ++i) wq3d::adj::body_adj_capture(i, F, a.W, a.D, h);
 #endif
 }
 
-void launch_adj_mark(wq3d_fan* f, int64_t h) {
-    FanAdjoint& a = *f->adjoint;
-#ifndef WQ3D_EMULATE
-    hipLaunchKernelGGL(k_adj_mark, dim3(1), dim3(64), 0, f->stream, a.D.ent_count, a.eoff, h);
-#else
-    a.eoff[h] = *a.D.ent_count;
-#endif
-}
-
-void launch_adj_scatter(wq3d_fan* f, int64_t h) {
-    FanAdjoint& a = *f->adjoint;
-#ifndef WQ3D_EMULATE
-    hipLaunchKernelGGL(k_adj_scatter, dim3(64), dim3(kAdjBlock), 0, f->stream, a.D, (const unsigned long long*)a.eoff, h);
-#else
-    for (unsigned long long k = a.eoff[h - 1]; k < a.eoff[h]; ++k) {
-        emu_range(a.D.adj + a.D.ent_idx[k], 1, "adjoint entry");
-        a.D.adj[a.D.ent_idx[k]] += a.D.ent_val[k];
-    }
-#endif
-}
-
-int64_t seed_tapes() { return (int64_t)(kTapePoolBytes / (sizeof(wq3d::ad::Node) + sizeof(double)) / kTapeSeed); }
-int64_t small_tapes() { return (int64_t)(kTapePoolBytes / (sizeof(wq3d::ad::Node) + sizeof(double)) / kTapeSmall); }
-
-void launch_adj_seed(wq3d_fan* f, const Slots& S, double time, int64_t lo, int64_t hi) {
+void launch_adj_seed(wq3d_fan* f, int64_t n) {
     FanAdjoint& a = *f->adjoint;
     AdjDev D = a.D;
     D.tape_cap = kTapeSeed;
 #ifndef WQ3D_EMULATE
-    const int64_t threads = std::min<int64_t>(seed_tapes(), hi - lo);
-    const unsigned grid = (unsigned)((threads + 63) / 64);
-    hipLaunchKernelGGL(k_adj_seed, dim3(grid), dim3(64), 0, f->stream, (const KParams*)a.kpw, S, time,
-                       (const AdjSeed*)a.seeds, (const double*)a.seed_int, (const double*)a.seed_grad, lo, hi, D);
+    const int64_t threads = std::min<int64_t>(tapes_of(kTapeSeed), (n + 63) & ~int64_t(63));
+    const unsigned grid = (unsigned)(threads / 64);
+    hipLaunchKernelGGL(k_adj_seed, dim3(grid), dim3(64), 0, f->stream, f->env->dev, f->F, (const Slots*)a.slots,
+                       (const double*)a.times, (const AdjSeed*)a.seeds, (const double*)a.seed_int,
+                       (const double*)a.seed_grad, n, D);
 #else
-    const KParams* P = a.kpw;
-    emu_check_launch(P->E, P->F, P->A, nullptr);
-    for (int64_t e = lo; e < hi; ++e)
-        wq3d::adj::body_adj_seed(e, 0, P->E, P->F, P->A, S, time, a.seeds, a.seed_int, a.seed_grad, D);
+    emu_check_launch(f->env->dev, f->F, a.W, nullptr);
+    for (int64_t e = 0; e < n; ++e)
+        wq3d::adj::body_adj_seed(e, 0, f->env->dev, f->F, a.slots, a.times, a.seeds, a.seed_int, a.seed_grad, D);
 #endif
 }
 
-void launch_adj_advance(wq3d_fan* f, const Slots& S, int64_t h) {
+// sort the n entries by key into skey / sval and find each ray's range
+wq3d_status sort_entries(wq3d_fan* f, int64_t n) {
     FanAdjoint& a = *f->adjoint;
-    const FanConst& F = f->F;
+    const int64_t N = f->F.N;
+    HIP_TRY(rt::memset_async(a.rbeg, 0, (size_t)N * sizeof(int64_t), f->stream));
+    HIP_TRY(rt::memset_async(a.rend, 0, (size_t)N * sizeof(int64_t), f->stream));
+    if (n == 0) return WQ3D_OK;
 #ifndef WQ3D_EMULATE
-    const unsigned grid = (unsigned)((F.N + kAdjBlock - 1) / kAdjBlock);
-    hipLaunchKernelGGL(k_adj_advance, dim3(grid), dim3(kAdjBlock), a.n_nodes * sizeof(double), f->stream,
-                       f->env->dev, F, S, h, a.D, (int32_t)a.n_nodes);
+    size_t bytes = a.sort_bytes;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(a.sort_tmp, bytes, a.D.ent_key, a.skey, a.D.ent_val, a.sval, (int)n,
+                                               0, 64, f->stream));
+    const unsigned grid = (unsigned)((n + kAdjBlock - 1) / kAdjBlock);
+    hipLaunchKernelGGL(k_adj_ranges, dim3(grid), dim3(kAdjBlock), 0, f->stream, (const unsigned long long*)a.skey, n,
+                       a.rbeg, a.rend);
 #else
-    for (int64_t k = 0; k < F.N; ++k) {
-        const int64_t i = emu_ray(k, F.N);
-        int k0;
-        double d0, d1;
-        wq3d::adj::body_adj_advance(i, f->env->dev, F, S, h, a.D, k0, d0, d1);
-        emu_range(a.D.node_grad + k0, 2, "node gradient");
-        a.D.node_grad[k0] += d0;
-        a.D.node_grad[k0 + 1] += d1;
+    std::vector<std::pair<unsigned long long, double>> e((size_t)n);
+    for (int64_t k = 0; k < n; ++k) e[(size_t)k] = {a.D.ent_key[k], a.D.ent_val[k]};
+    std::stable_sort(e.begin(), e.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    const int shift = 8 + wq3d::adj::kEntryStepBits;
+    for (int64_t k = 0; k < n; ++k) {
+        a.skey[k] = e[(size_t)k].first;
+        a.sval[k] = e[(size_t)k].second;
+    }
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t r = (int64_t)(a.skey[k] >> shift);
+        emu_range(a.rbeg + r, 1, "entry ray");
+        if (k == 0 || (int64_t)(a.skey[k - 1] >> shift) != r) a.rbeg[r] = k;
+        if (k == n - 1 || (int64_t)(a.skey[k + 1] >> shift) != r) a.rend[r] = k + 1;
     }
 #endif
+    return WQ3D_OK;
 }
 
-void launch_adj_reflect(wq3d_fan* f, const Slots& Sp, const Slots& Sq, int64_t hp, int64_t hq) {
+void launch_adj_sweep(wq3d_fan* f, int64_t K) {
     FanAdjoint& a = *f->adjoint;
     const FanConst& F = f->F;
     AdjDev D = a.D;
     D.tape_cap = kTapeSmall;
 #ifndef WQ3D_EMULATE
-    const int64_t threads = std::min<int64_t>(small_tapes(), F.N);
-    const unsigned grid = (unsigned)((threads + 63) / 64);
-    hipLaunchKernelGGL(k_adj_reflect, dim3(grid), dim3(64), 0, f->stream, f->env->dev, F, Sp, Sq, hp, hq, D);
+    const int64_t threads = std::min<int64_t>(tapes_of(kTapeSmall), (F.N + kAdjBlock - 1) / kAdjBlock * kAdjBlock);
+    const unsigned grid = (unsigned)((threads + kAdjBlock - 1) / kAdjBlock);
+    hipLaunchKernelGGL(k_adj_sweep, dim3(grid), dim3(kAdjBlock), a.n_nodes * sizeof(double), f->stream, f->env->dev,
+                       F, (const Slots*)a.slots, K, (const unsigned long long*)a.skey, (const double*)a.sval,
+                       (const int64_t*)a.rbeg, (const int64_t*)a.rend, D, (int32_t)a.n_nodes);
 #else
+    auto add = [&](int k0, double d0, double d1) {
+        emu_range(a.D.node_grad + k0, 2, "node gradient");
+        a.D.node_grad[k0] += d0;
+        a.D.node_grad[k0 + 1] += d1;
+    };
     for (int64_t k = 0; k < F.N; ++k) {
         const int64_t i = emu_ray(k, F.N);
-        if (a.hflag[hp * F.N + i] != 0) wq3d::adj::body_adj_reflect(i, 0, f->env->dev, F, Sp, Sq, hp, hq, D);
+        wq3d::adj::body_adj_sweep(i, 0, f->env->dev, F, a.slots, K, a.skey, a.sval, a.rbeg[i], a.rend[i], D, add);
     }
 #endif
 }
@@ -349,8 +373,8 @@ void launch_adj_init(wq3d_fan* f, const Slots& S) {
     AdjDev D = a.D;
     D.tape_cap = kTapeSmall;
 #ifndef WQ3D_EMULATE
-    const int64_t threads = std::min<int64_t>(small_tapes(), F.N);
-    const unsigned grid = (unsigned)((threads + 63) / 64);
+    const int64_t threads = std::min<int64_t>(tapes_of(kTapeSmall), (F.N + 63) & ~int64_t(63));
+    const unsigned grid = (unsigned)(threads / 64);
     hipLaunchKernelGGL(k_adj_init, dim3(grid), dim3(64), 0, f->stream, f->env->dev, F, S, D);
 #else
     for (int64_t k = 0; k < F.N; ++k) wq3d::adj::body_adj_init(emu_ray(k, F.N), 0, f->env->dev, F, S, D);
@@ -373,9 +397,9 @@ wq3d_status wq3d_adjoint_checkpoint(wq3d_fan* f) {
                     "the adjoint covers a 1-D linear sound speed grid between flat boundaries");
     if (f->F.spreading != WQ3D_SPREADING_HYBRID_GAUSSIAN)
         return fail(WQ3D_ERR_UNSUPPORTED, "the adjoint covers hybrid Gaussian spreading");
-    HIP_TRY(rt::set_device(f->device));
     if (f->env->dev.profile.grid.ax[0].n > 4096)
         return fail(WQ3D_ERR_UNSUPPORTED, "the adjoint covers sound speed grids of at most 4096 nodes");
+    HIP_TRY(rt::set_device(f->device));
     if (!f->adjoint) {
         if (f->steps != 0) return fail(WQ3D_ERR_STATE, "the first checkpoint must be taken before the first step");
         f->adjoint = new FanAdjoint();
@@ -383,7 +407,8 @@ wq3d_status wq3d_adjoint_checkpoint(wq3d_fan* f) {
     }
     FanAdjoint& a = *f->adjoint;
     if (!a.cps.empty() && a.cps.back().steps == f->steps) return WQ3D_OK;
-    if (!a.cps.empty() && a.cps.back().steps > f->steps) return fail(WQ3D_ERR_STATE, "checkpoints must advance");
+    if (!a.cps.empty() && f->steps - a.cps.back().steps >= (int64_t(1) << wq3d::adj::kEntryStepBits))
+        return fail(WQ3D_ERR_ARGUMENT, "checkpoints must be less than 65536 steps apart");
     AdjCheckpoint cp;
     cp.steps = f->steps;
     cp.time = f->time;
@@ -413,6 +438,8 @@ wq3d_status wq3d_adjoint_gradient(wq3d_fan* f, const wq3d_eigenray* rays, const 
         return fail(WQ3D_ERR_STATE, "no checkpoint at step 0 (wq3d_adjoint_checkpoint before stepping)");
     FanAdjoint& a = *f->adjoint;
     if (num_nodes != a.n_nodes) return fail(WQ3D_ERR_ARGUMENT, "num_nodes is not the sound speed grid's size");
+    if (f->steps - a.cps.back().steps >= (int64_t(1) << wq3d::adj::kEntryStepBits))
+        return fail(WQ3D_ERR_ARGUMENT, "checkpoints must be less than 65536 steps apart");
     HIP_TRY(rt::set_device(f->device));
     const FanConst& F = f->F;
     const int nf = F.nf;
@@ -453,9 +480,9 @@ wq3d_status wq3d_adjoint_gradient(wq3d_fan* f, const wq3d_eigenray* rays, const 
     size_t next_ray = order.size();  // eigenrays are consumed from the back, segment by segment
     for (size_t jj = ns; jj-- > 0;) {
         const AdjCheckpoint& cp = a.cps[jj];
-        const int64_t a0 = cp.steps, e0 = seg_end[jj];
-        if (e0 <= a0) continue;
-        // this segment's eigenrays, uploaded by step
+        const int64_t a0 = cp.steps, e0 = seg_end[jj], K = e0 - a0;
+        if (K <= 0) continue;
+        // this segment's eigenrays
         size_t first = next_ray;
         while (first > 0 && rays[order[first - 1]].step > a0) --first;
         const int64_t nseg = (int64_t)(next_ray - first);
@@ -469,57 +496,60 @@ wq3d_status wq3d_adjoint_gradient(wq3d_fan* f, const wq3d_eigenray* rays, const 
                 return fail(WQ3D_ERR_DEVICE, "adjoint seed allocation failed");
             }
         }
-        std::vector<int64_t> call_lo((size_t)(e0 - a0 + 2), nseg);
-        {
+        if (nseg) {
             std::vector<AdjSeed> hs((size_t)nseg);
             std::vector<double> hi((size_t)(nseg * nf)), hg((size_t)(nseg * nf));
-            for (int64_t k = nseg - 1; k >= 0; --k) {
+            for (int64_t k = 0; k < nseg; ++k) {
                 const int64_t r = order[first + (size_t)k];
                 const wq3d_eigenray& x = rays[r];
                 hs[(size_t)k] = AdjSeed{(int64_t)x.t1 * F.nt2 + x.t2, (int64_t)x.de * F.naz + x.az, x.travel_time,
-                                        grad_time[r]};
+                                        grad_time[r], x.step - a0};
                 for (int q = 0; q < nf; ++q) {
                     hi[(size_t)(k * nf + q)] = intensity[r * nf + q];
                     hg[(size_t)(k * nf + q)] = grad_intensity[r * nf + q];
                 }
-                call_lo[(size_t)(x.step - a0)] = k;
             }
-            for (int64_t h = e0 - a0; h >= 1; --h) call_lo[(size_t)h] = std::min(call_lo[(size_t)h], call_lo[(size_t)h + 1]);
-            if (nseg) {
-                HIP_TRY(rt::memcpy(a.seeds, hs.data(), hs.size() * sizeof(AdjSeed), hipMemcpyHostToDevice));
-                HIP_TRY(rt::memcpy(a.seed_int, hi.data(), hi.size() * sizeof(double), hipMemcpyHostToDevice));
-                HIP_TRY(rt::memcpy(a.seed_grad, hg.data(), hg.size() * sizeof(double), hipMemcpyHostToDevice));
-            }
+            HIP_TRY(rt::memcpy(a.seeds, hs.data(), hs.size() * sizeof(AdjSeed), hipMemcpyHostToDevice));
+            HIP_TRY(rt::memcpy(a.seed_int, hi.data(), hi.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIP_TRY(rt::memcpy(a.seed_grad, hg.data(), hg.size() * sizeof(double), hipMemcpyHostToDevice));
         }
         next_ray = first;
-        // replay the segment, keeping its windows and the seeds of its steps
+        // 1. replay the segment into the history
         st = copy_state(a.ws_mem, F, a.W, cp.A, f->stream, false);
         if (st) return st;
-        HIP_TRY(rt::memset_async(D.ent_count, 0, sizeof(unsigned long long), f->stream));
-        HIP_TRY(rt::memset_async(a.eoff, 0, sizeof(unsigned long long), f->stream));
-        Replay rp{f, {cp.phys[0], cp.phys[1], cp.phys[2], cp.phys[3]}, cp.time, cp.steps};
-        std::vector<Slots> slots_at((size_t)(e0 - a0 + 1));
+        Replay rp{{cp.phys[0], cp.phys[1], cp.phys[2], cp.phys[3]}, cp.time, cp.steps};
+        std::vector<Slots> slots_at((size_t)(K + 1));
+        std::vector<double> times_at((size_t)(K + 1));
         slots_at[0] = rp.slots();
+        times_at[0] = rp.time;
         launch_adj_capture(f, 0);
-        for (int64_t h = 1; h <= e0 - a0; ++h) {
+        for (int64_t h = 1; h <= K; ++h) {
             launch_bounce(f->stream, grid, a.kpw, E, Fw, a.W, rp.slots(), a.Rw, rp.time, rp.steps);
             rp.rotate();
             rp.time += F.dt;
             rp.steps += 1;
             launch_advance(f->stream, grid, E, Fw, a.W, rp.slots());
-            const int64_t lo = call_lo[(size_t)h], hi_ = call_lo[(size_t)h + 1];
-            if (hi_ > lo) launch_adj_seed(f, rp.slots(), rp.time, lo, hi_);
-            launch_adj_mark(f, h);
             launch_adj_capture(f, h);
             slots_at[(size_t)h] = rp.slots();
+            times_at[(size_t)h] = rp.time;
         }
         HIP_TRY(rt::last_error());
-        // the step adjoints, last step first
-        for (int64_t h = e0 - a0; h >= 1; --h) {
-            launch_adj_scatter(f, h);
-            launch_adj_advance(f, slots_at[(size_t)h], h);
-            launch_adj_reflect(f, slots_at[(size_t)h - 1], slots_at[(size_t)h], h - 1, h);
-        }
+        HIP_TRY(rt::memcpy_async(a.slots, slots_at.data(), slots_at.size() * sizeof(Slots), hipMemcpyHostToDevice,
+                                 f->stream));
+        HIP_TRY(rt::memcpy_async(a.times, times_at.data(), times_at.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 f->stream));
+        // 2. the seeds of the segment's eigenrays, 3. sorted by ray and step
+        HIP_TRY(rt::memset_async(D.ent_count, 0, sizeof(unsigned long long), f->stream));
+        if (nseg) launch_adj_seed(f, nseg);
+        HIP_TRY(rt::last_error());
+        unsigned long long n_ent = 0;
+        HIP_TRY(rt::sync(f->stream));  // the fan's stream does not wait for the legacy stream's copies
+        HIP_TRY(rt::memcpy(&n_ent, D.ent_count, sizeof(n_ent), hipMemcpyDeviceToHost));
+        n_ent = std::min<unsigned long long>(n_ent, (unsigned long long)kEntriesPerSegment);
+        st = sort_entries(f, (int64_t)n_ent);
+        if (st) return st;
+        // 4. every ray backward through the segment
+        launch_adj_sweep(f, K);
         HIP_TRY(rt::last_error());
         HIP_TRY(rt::sync(f->stream));
     }
