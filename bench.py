@@ -284,6 +284,62 @@ def leg_sources(lib, n_sources):
             "batched_ray_steps_per_s": ray_steps / batched, "speedup": alone / batched, "shape": shape}
 
 
+def leg_adjoint(lib, n_profiles, workers):
+    """North_star's differentiation form on config 5: the forward runs of
+    n_profiles seeded profiles as one fan_batch with a checkpoint of every
+    fan every 100 steps, then the gradient of a loss over every eigenray's
+    travel time and intensities with respect to each profile's 64 node
+    speeds (wq3d_adjoint_gradient, `workers` fans at once).  Profile 0 alone
+    too: its plain forward run and its backward pass.  The gradient's
+    agreement with central differences of the oracle is checked in
+    tests/test_adjoint.py."""
+    import numpy as np
+    from underseamodelinglibrary_amd import scenarios
+    from underseamodelinglibrary_amd.usml.adjoint import profile_runs
+    from underseamodelinglibrary_amd.usml.waveq3d import wave_queue
+
+    def make(speeds):
+        return scenarios.config5(speeds=speeds)
+
+    speeds = np.stack([scenarios.config5_profile(p)[1] for p in range(n_profiles)])
+    runs = profile_runs(make, checkpoint_every=100, batched=True, lib=lib)
+    t0 = time.perf_counter()
+    runs.run(speeds)
+    fwd = time.perf_counter() - t0
+    counts = runs.counts()
+    nf = runs.nf
+    rng = np.random.default_rng(1)
+    gt = rng.normal(size=sum(counts))
+    gi = rng.normal(size=(sum(counts), nf))
+    t0 = time.perf_counter()
+    grad = runs.gradient(gt, gi, workers=workers, release=True)
+    bwd = time.perf_counter() - t0
+    runs.close()
+    # one profile alone: plain forward (no checkpoints), then with
+    # checkpoints and its backward pass
+    sc = make(speeds[0])
+    q = wave_queue(*sc.args(), lib=lib)
+    t0 = time.perf_counter()
+    lib.check(lib.wq3d_step(q._fan, sc.steps))
+    q.sync()
+    one_fwd = time.perf_counter() - t0
+    q.close()
+    one = profile_runs(make, checkpoint_every=100, batched=False, lib=lib)
+    one.run(speeds[:1])
+    n1 = one.counts()[0]
+    t0 = time.perf_counter()
+    one.gradient(rng.normal(size=n1), rng.normal(size=(n1, nf)), workers=1, release=True)
+    one_bwd = time.perf_counter() - t0
+    one.close()
+    return {"workload": f"config 5 differentiation: {n_profiles} profiles x 64 node speeds, loss over every "
+                        "eigenray's travel time and 4 intensities (dB), checkpoints every 100 steps",
+            "forward_batched_s": fwd, "backward_s": bwd, "backward_workers": workers,
+            "backward_over_forward": bwd / fwd, "eigenrays": int(sum(counts)),
+            "grad_abs_max": float(np.abs(grad).max()),
+            "one_profile": {"forward_plain_s": one_fwd, "backward_s": one_bwd, "backward_over_forward":
+                            one_bwd / one_fwd, "eigenrays": int(n1)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,7 +350,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--legs", default="config1,config2fan,collection,sources")
+    ap.add_argument("--legs", default="config1,config2fan,collection,sources,adjoint")
+    ap.add_argument("--adjoint-workers", type=int, default=4)
     ap.add_argument("--sources", type=int, default=64, help="config 4 sources in the batch leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -379,6 +436,10 @@ def main():
     if "sources" in names and args.sources > 0:
         legs["config4_sources"] = leg_sources(lib, args.sources)
         log(f"[bench] config 4 sources: {legs['config4_sources']['speedup']:.2f}x batched over alone")
+    if "adjoint" in names:
+        legs["adjoint"] = leg_adjoint(lib, n, args.adjoint_workers)
+        log(f"[bench] adjoint: forward {legs['adjoint']['forward_batched_s']:.2f} s, backward "
+            f"{legs['adjoint']['backward_s']:.2f} s; one profile {legs['adjoint']['one_profile']['backward_over_forward']:.2f}x")
     out["legs"] = legs
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle)")

@@ -115,6 +115,32 @@ __host__ __device__ inline void reverse(Tape* t) {
     }
 }
 
+// A value computed outside the tape from n inputs, with its partial
+// derivatives d value / d x[k] given: recorded as a chain of n - 1 two-input
+// operations instead of the operations of its formula.
+template <int NI>
+__host__ __device__ __forceinline__ V combine(double v, const V (&x)[NI], const double (&d)[NI]) {
+    Tape* t = nullptr;
+    for (int k = 0; k < NI; ++k)
+        if (x[k].i >= 0) t = x[k].t;
+    if (!t) return V(v);
+    int32_t acc = -1;
+    double dacc = 0.0;
+    for (int k = 0; k < NI; ++k) {
+        if (x[k].i < 0 || d[k] == 0.0) continue;
+        if (acc < 0) {
+            acc = x[k].i;
+            dacc = d[k];
+            continue;
+        }
+        acc = push(t, acc, dacc, x[k].i, d[k]);
+        dacc = 1.0;
+        if (acc < 0) return V(v);
+    }
+    if (acc < 0) return V(v);
+    return V(v, push(t, acc, dacc, kNone, 0.0), t);
+}
+
 // ------------------------------------------------------------ arithmetic
 __host__ __device__ __forceinline__ V operator+(const V& x, const V& y) { return rec2(x, y, x.v + y.v, 1.0, 1.0); }
 __host__ __device__ __forceinline__ V operator-(const V& x, const V& y) { return rec2(x, y, x.v - y.v, 1.0, -1.0); }

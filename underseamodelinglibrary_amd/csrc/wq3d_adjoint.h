@@ -607,6 +607,45 @@ WQ_HD_INLINE bool offsets_core_t(const T D[3][3][3], const double delta[3], bool
     return unstable;
 }
 
+// point_distance and target_distance2 on tape values, each recorded as one
+// combined value of its inputs with partials derived from its formula (the
+// value itself is the double function's, bit for bit)
+WQ_HD_INLINE V point_distance_v(const V& r1, const V& t1, const V& p1, const V& r2, const V& t2, const V& p2) {
+    const double L = point_distance(r1.v, t1.v, p1.v, r2.v, t2.v, p2.v);
+    const double ht = 0.5 * (t1.v - t2.v), hp = 0.5 * (p1.v - p2.v);
+    const double st = wqm::sin(ht), sp = wqm::sin(hp);
+    const double s1 = wqm::sin(t1.v), s2 = wqm::sin(t2.v);
+    const double dn = 1.0 - 2.0 * (st * st + s1 * s2 * sp * sp);
+    const double q = r1.v * r1.v + r2.v * r2.v - 2.0 * (r1.v * r2.v * dn);
+    const double dq = L > 0.0 ? (q < 0.0 ? -0.5 : 0.5) / L : 0.0;   // dL / dq
+    const double ddn = -2.0 * r1.v * r2.v * dq;                      // dL / d dotnorm
+    const double dst = -4.0 * st * ddn, dsp = -4.0 * s1 * s2 * sp * ddn;
+    const double cht = 0.5 * wqm::cos(ht), chp = 0.5 * wqm::cos(hp);
+    const double x[6] = {(2.0 * r1.v - 2.0 * r2.v * dn) * dq,
+                         dst * cht - 2.0 * wqm::cos(t1.v) * s2 * sp * sp * ddn,
+                         dsp * chp,
+                         (2.0 * r2.v - 2.0 * r1.v * dn) * dq,
+                         -dst * cht - 2.0 * s1 * wqm::cos(t2.v) * sp * sp * ddn,
+                         -dsp * chp};
+    const V in[6] = {r1, t1, p1, r2, t2, p2};
+    return ad::combine<6>(L, in, x);
+}
+
+WQ_HD_INLINE V target_distance2_v(const V& rho, const V& theta, const V& phi, const V& st, double Rt, double Tt,
+                                  double Pt, double St) {
+    const double D = target_distance2(rho.v, theta.v, phi.v, st.v, Rt, Tt, Pt, St);
+    const double h = 0.5 * (theta.v - Tt);
+    const double g = 0.5 * (phi.v - Pt);
+    const double inner = 1.0 - 2.0 * (h * h + St * (st.v * (g * g)));
+    const double q = rho.v * rho.v + Rt * Rt - 2.0 * Rt * (rho.v * inner);
+    const double sq = q < 0.0 ? -1.0 : 1.0;                          // dD / dq
+    const double dinner = -2.0 * Rt * rho.v * sq;
+    const double x[4] = {(2.0 * rho.v - 2.0 * Rt * inner) * sq, dinner * (-4.0 * h) * 0.5,
+                         dinner * (-4.0 * St * st.v * g) * 0.5, dinner * (-2.0 * St * g * g)};
+    const V in[4] = {rho, theta, phi, st};
+    return ad::combine<4>(D, in, x);
+}
+
 // What an eigenray's construction reads from the fan, as tape inputs (or
 // plain values): target-distance snapshots, positions and attenuations of
 // (slot, ray), each tagged with its adjoint field.
@@ -625,7 +664,7 @@ struct Reader {
 WQ_HD_INLINE V ray_dist_v(const Reader& rd, int slot, int d1, int a1, int d2, int a2) {
     const int64_t i1 = (int64_t)d1 * rd.F.naz + a1;
     const int64_t i2 = (int64_t)d2 * rd.F.naz + a2;
-    return point_distance_t(rd.get(rd.A.pr, kPr, slot, i1), rd.get(rd.A.pt, kPt, slot, i1),
+    return point_distance_v(rd.get(rd.A.pr, kPr, slot, i1), rd.get(rd.A.pt, kPt, slot, i1),
                             rd.get(rd.A.pp, kPp, slot, i1), rd.get(rd.A.pr, kPr, slot, i2),
                             rd.get(rd.A.pt, kPt, slot, i2), rd.get(rd.A.pp, kPp, slot, i2));
 }
@@ -695,10 +734,14 @@ struct SpreadV {
 };
 
 WQ_HD_INLINE void gaussian_add_v(const SpreadV& s, const V& d, const V& w, double A, V* out, bool assign) {
-    using ad::sqrt;
     for (int k = 0; k < s.nf; ++k) {
-        const V bw = s.spread[k] + 2.0 * 2.0 * w * w;
-        const V g = wqm::exp((-0.5 * d * d) / bw) / sqrt(bw) * A;
+        // gaussian_add's value, and its partials in d, w and the spread
+        const double bw = s.spread[k].v + 2.0 * 2.0 * w.v * w.v;
+        const double gv = wqm::exp((-0.5 * d.v * d.v) / bw) / ::sqrt(bw) * A;
+        const double dbw = gv * (0.5 * d.v * d.v / (bw * bw) - 0.5 / bw);
+        const double x[3] = {-gv * d.v / bw, dbw * 8.0 * w.v, dbw};
+        const V in[3] = {d, w, s.spread[k]};
+        const V g = ad::combine<3>(gv, in, x);
         out[k] = assign ? g : out[k] + g;
     }
 }
@@ -851,7 +894,7 @@ WQ_HD bool eigenray_v(const DEnv& E, const FanConst& F, const Arrays& A, const S
         const int s = k / 9, nde = (k / 3) % 3, naz = k % 3;
         const int64_t j = cube_ray(F, de, az, nde, naz);
         const int slot = sl[s];
-        D[s][nde][naz] = target_distance2_t(rd.get(A.tr, kSr, slot, j), rd.get(A.tt, kSt, slot, j),
+        D[s][nde][naz] = target_distance2_v(rd.get(A.tr, kSr, slot, j), rd.get(A.tt, kSt, slot, j),
                                             rd.get(A.tp, kSp, slot, j), rd.get(A.ts, kSs, slot, j), F.tgt_r[t],
                                             F.tgt_t[t], F.tgt_p[t], F.tgt_s[t]);
         Dv[s][nde][naz] = D[s][nde][naz].v;

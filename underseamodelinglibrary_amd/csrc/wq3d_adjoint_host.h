@@ -20,6 +20,8 @@
 #include <hipcub/hipcub.hpp>
 #endif
 
+#include <memory>
+
 #include "wq3d_adjoint.h"
 
 namespace {
@@ -32,7 +34,7 @@ constexpr int32_t kTapeSeed = 16384;    // operations of an eigenray's tape
 constexpr int kAdjBlock = 256;
 
 #ifndef WQ3D_EMULATE
-constexpr size_t kTapePoolBytes = size_t(2) << 30;
+constexpr size_t kTapePoolBytes = size_t(4) << 30;
 constexpr int64_t kEntriesPerSegment = int64_t(1) << 25;
 #else
 constexpr size_t kTapePoolBytes = size_t(16) << 20;
@@ -48,30 +50,52 @@ struct AdjCheckpoint {
 
 }  // namespace
 
-struct FanAdjoint {
-    DeviceBuffers cp_mem;  // checkpoints
-    std::vector<AdjCheckpoint> cps;
-    // backward workspace
-    DeviceBuffers ws_mem;
-    bool have_ws = false;
-    int64_t hist_calls = 0;
+// A backward pass's device workspace (replay arrays, history, tapes,
+// entries).  Workspaces go back to a per-process pool when a fan releases
+// its adjoint and are reused by the next fan of the same shape, so the
+// backward passes of many fans allocate and free nothing after the first
+// (hipFree would synchronise the device and stop passes from overlapping).
+struct AdjWorkspace {
+    DeviceBuffers mem;
+    int device = 0;
+    int64_t N = 0, nf = 0, n_nodes = 0, calls = 0;
     Arrays W{};                 // replay arrays
     KParams* kpw = nullptr;     // {E, F without records, W, Rw}
     Records Rw{};
     AdjDev D{};
-    Slots* slots = nullptr;     // [hist_calls] slots after each replayed step
-    double* times = nullptr;    // [hist_calls] time after each replayed step
-    unsigned long long *skey = nullptr;  // sorted entries
+    Slots* slots = nullptr;     // [calls] slots after each replayed step
+    double* times = nullptr;    // [calls] time after each replayed step
+    unsigned long long* skey = nullptr;  // sorted entries
     double* sval = nullptr;
     int64_t *rbeg = nullptr, *rend = nullptr;  // [N] each ray's range of sorted entries
     void* sort_tmp = nullptr;
     size_t sort_bytes = 0;
+};
+
+struct FanAdjoint {
+    DeviceBuffers cp_mem;  // checkpoints
+    std::vector<AdjCheckpoint> cps;
+    std::unique_ptr<AdjWorkspace> ws;
     AdjSeed* seeds = nullptr;
     double *seed_int = nullptr, *seed_grad = nullptr;
     int64_t seed_cap = 0;
     DeviceBuffers seed_mem;
     int64_t n_nodes = 0;
 };
+
+namespace {
+
+constexpr size_t kPooledWorkspaces = 8;
+std::mutex& ws_pool_mutex() {
+    static std::mutex m;
+    return m;
+}
+std::vector<std::unique_ptr<AdjWorkspace>>& ws_pool() {
+    static std::vector<std::unique_ptr<AdjWorkspace>> p;
+    return p;
+}
+
+}  // namespace
 
 namespace {
 
@@ -177,23 +201,22 @@ wq3d_status copy_state(DeviceBuffers& mem, const FanConst& F, Arrays& dst, const
     return WQ3D_OK;
 }
 
-wq3d_status adjoint_workspace(wq3d_fan* f, int64_t calls) {
-    FanAdjoint& a = *f->adjoint;
+// a new workspace for fans of f's shape with a history of `calls` steps
+wq3d_status new_workspace(wq3d_fan* f, int64_t calls, std::unique_ptr<AdjWorkspace>& out) {
     const FanConst& F = f->F;
-    if (a.have_ws && a.hist_calls >= calls) return WQ3D_OK;
-    a.ws_mem.reset();
-    a.have_ws = false;
-    DeviceBuffers& m = a.ws_mem;
+    auto w = std::make_unique<AdjWorkspace>();
+    w->device = f->device;
+    w->N = F.N;
+    w->nf = F.nf;
+    w->n_nodes = f->adjoint->n_nodes;
+    w->calls = calls;
+    DeviceBuffers& m = w->mem;
     const int64_t N = F.N;
-    // replay arrays (the search record stays the fan's: k_advance rewrites it
-    // every step before k_detect reads it)
-    if (copy_state(m, F, a.W, f->A, f->stream, true)) return WQ3D_ERR_DEVICE;
-    a.W.win = f->A.win;
-    a.W.tsup = f->A.tsup;
     auto nomem = [&](const char* what) {
         return fail(WQ3D_ERR_DEVICE, std::string("adjoint workspace allocation failed (") + what + ", history of " +
                                          std::to_string(calls) + " steps)");
     };
+    if (copy_state(m, F, w->W, f->A, f->stream, true)) return WQ3D_ERR_DEVICE;
     {
         unsigned long long* c = nullptr;
         int32_t* e = nullptr;
@@ -201,7 +224,7 @@ wq3d_status adjoint_workspace(wq3d_fan* f, int64_t calls) {
             rt::memset(c, 0, (6 + 2 * kScanSlots) * sizeof(unsigned long long)) != hipSuccess ||
             rt::memset(e, 0, sizeof(int32_t)) != hipSuccess)
             return nomem("records");
-        Records& R = a.Rw;
+        Records& R = w->Rw;
         R = Records{};
         R.ray_count = c;
         R.verb_count = c + 1;
@@ -213,16 +236,8 @@ wq3d_status adjoint_workspace(wq3d_fan* f, int64_t calls) {
         R.error = e;
         R.cand_cap = 0;
     }
-    FanConst Fw = F;
-    Fw.want_verbs = 0;
-    Fw.want_refl = 0;
-    if (m.alloc(&a.kpw, 1) != hipSuccess) return nomem("parameters");
-    {
-        const KParams kp{f->env->dev, Fw, a.W, a.Rw};
-        if (rt::memcpy(a.kpw, &kp, sizeof(KParams), hipMemcpyHostToDevice) != hipSuccess)
-            return fail(WQ3D_ERR_DEVICE, "adjoint parameter upload failed");
-    }
-    AdjDev& D = a.D;
+    if (m.alloc(&w->kpw, 1) != hipSuccess) return nomem("parameters");
+    AdjDev& D = w->D;
     D = AdjDev{};
     D.N = N;
     D.nf = F.nf;
@@ -232,7 +247,7 @@ wq3d_status adjoint_workspace(wq3d_fan* f, int64_t calls) {
     double *hist = nullptr, *hatt = nullptr;
     int32_t* hint = nullptr;
     uint8_t *hedge = nullptr, *hflag = nullptr;
-    if (m.alloc(&D.adj, nadj) != hipSuccess || m.alloc(&D.node_grad, std::max<int64_t>(1, a.n_nodes)) != hipSuccess)
+    if (m.alloc(&D.adj, nadj) != hipSuccess || m.alloc(&D.node_grad, std::max<int64_t>(1, w->n_nodes)) != hipSuccess)
         return nomem("adjoint state");
     if (m.alloc(&hist, (size_t)calls * wq3d::adj::kAtt * n4) != hipSuccess ||
         m.alloc(&hatt, (size_t)calls * F.nf * n4) != hipSuccess || m.alloc(&hint, (size_t)calls * 5 * n4) != hipSuccess ||
@@ -241,19 +256,19 @@ wq3d_status adjoint_workspace(wq3d_fan* f, int64_t calls) {
     if (m.alloc(&D.tape_node, tape_nodes) != hipSuccess || m.alloc(&D.tape_adj, tape_nodes) != hipSuccess)
         return nomem("tapes");
     if (m.alloc(&D.ent_key, kEntriesPerSegment) != hipSuccess || m.alloc(&D.ent_val, kEntriesPerSegment) != hipSuccess ||
-        m.alloc(&a.skey, kEntriesPerSegment) != hipSuccess || m.alloc(&a.sval, kEntriesPerSegment) != hipSuccess ||
-        m.alloc(&a.rbeg, N) != hipSuccess || m.alloc(&a.rend, N) != hipSuccess ||
+        m.alloc(&w->skey, kEntriesPerSegment) != hipSuccess || m.alloc(&w->sval, kEntriesPerSegment) != hipSuccess ||
+        m.alloc(&w->rbeg, N) != hipSuccess || m.alloc(&w->rend, N) != hipSuccess ||
         m.alloc(&D.ent_count, 1) != hipSuccess || m.alloc(&D.err, 1) != hipSuccess)
         return nomem("seed entries");
-    if (m.alloc(&a.slots, calls) != hipSuccess || m.alloc(&a.times, calls) != hipSuccess) return nomem("steps");
+    if (m.alloc(&w->slots, calls) != hipSuccess || m.alloc(&w->times, calls) != hipSuccess) return nomem("steps");
 #ifndef WQ3D_EMULATE
-    a.sort_bytes = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, a.sort_bytes, D.ent_key, a.skey, D.ent_val, a.sval,
+    w->sort_bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, w->sort_bytes, D.ent_key, w->skey, D.ent_val, w->sval,
                                            (int)kEntriesPerSegment, 0, 64, f->stream) != hipSuccess)
         return fail(WQ3D_ERR_DEVICE, "radix sort set-up failed");
     uint8_t* tmp = nullptr;
-    if (m.alloc(&tmp, a.sort_bytes) != hipSuccess) return nomem("sort");
-    a.sort_tmp = tmp;
+    if (m.alloc(&tmp, w->sort_bytes) != hipSuccess) return nomem("sort");
+    w->sort_tmp = tmp;
 #endif
     D.hist = hist;
     D.hatt = hatt;
@@ -261,9 +276,53 @@ wq3d_status adjoint_workspace(wq3d_fan* f, int64_t calls) {
     D.hedge = hedge;
     D.hflag = hflag;
     D.ent_cap = kEntriesPerSegment;
-    a.hist_calls = calls;
-    a.have_ws = true;
+    out = std::move(w);
     return WQ3D_OK;
+}
+
+// give f a workspace with a history of at least `calls` steps (its own, one
+// from the pool, or a new one) and point it at f's parameters
+wq3d_status adjoint_workspace(wq3d_fan* f, int64_t calls) {
+    FanAdjoint& a = *f->adjoint;
+    const FanConst& F = f->F;
+    auto fits = [&](const AdjWorkspace& w) {
+        return w.device == f->device && w.N == F.N && w.nf == F.nf && w.n_nodes == a.n_nodes && w.calls >= calls;
+    };
+    if (a.ws && !fits(*a.ws)) a.ws.reset();
+    if (!a.ws) {
+        std::lock_guard<std::mutex> lock(ws_pool_mutex());
+        auto& pool = ws_pool();
+        for (size_t k = 0; k < pool.size(); ++k)
+            if (fits(*pool[k])) {
+                a.ws = std::move(pool[k]);
+                pool.erase(pool.begin() + (long)k);
+                break;
+            }
+    }
+    if (!a.ws) {
+        wq3d_status st = new_workspace(f, calls, a.ws);
+        if (st) return st;
+    }
+    AdjWorkspace& w = *a.ws;
+    // the search record stays the fan's: k_advance rewrites it every step
+    // before k_detect reads it
+    w.W.win = f->A.win;
+    w.W.tsup = f->A.tsup;
+    FanConst Fw = F;
+    Fw.want_verbs = 0;
+    Fw.want_refl = 0;
+    const KParams kp{f->env->dev, Fw, w.W, w.Rw};
+    if (rt::memcpy(w.kpw, &kp, sizeof(KParams), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(WQ3D_ERR_DEVICE, "adjoint parameter upload failed");
+    return WQ3D_OK;
+}
+
+// a fan's workspace goes back to the pool (or is freed when the pool is full)
+void give_workspace(FanAdjoint& a) {
+    if (!a.ws) return;
+    std::lock_guard<std::mutex> lock(ws_pool_mutex());
+    if (ws_pool().size() < kPooledWorkspaces) ws_pool().push_back(std::move(a.ws));
+    a.ws.reset();
 }
 
 // the replayed fan's slot order, time and step count
@@ -286,27 +345,27 @@ void launch_adj_capture(wq3d_fan* f, int64_t h) {
     const FanConst& F = f->F;
 #ifndef WQ3D_EMULATE
     const unsigned grid = (unsigned)((F.N + kAdjBlock - 1) / kAdjBlock);
-    hipLaunchKernelGGL(k_adj_capture, dim3(grid), dim3(kAdjBlock), 0, f->stream, F, a.W, a.D, h);
+    hipLaunchKernelGGL(k_adj_capture, dim3(grid), dim3(kAdjBlock), 0, f->stream, F, a.ws->W, a.ws->D, h);
 #else
-    emu_check_arrays(F, a.W);
-    for (int64_t i = 0; i < F.N; ++i) wq3d::adj::body_adj_capture(i, F, a.W, a.D, h);
+    emu_check_arrays(F, a.ws->W);
+    for (int64_t i = 0; i < F.N; ++i) wq3d::adj::body_adj_capture(i, F, a.ws->W, a.ws->D, h);
 #endif
 }
 
 void launch_adj_seed(wq3d_fan* f, int64_t n) {
     FanAdjoint& a = *f->adjoint;
-    AdjDev D = a.D;
+    AdjDev D = a.ws->D;
     D.tape_cap = kTapeSeed;
 #ifndef WQ3D_EMULATE
     const int64_t threads = std::min<int64_t>(tapes_of(kTapeSeed), (n + 63) & ~int64_t(63));
     const unsigned grid = (unsigned)(threads / 64);
-    hipLaunchKernelGGL(k_adj_seed, dim3(grid), dim3(64), 0, f->stream, f->env->dev, f->F, (const Slots*)a.slots,
-                       (const double*)a.times, (const AdjSeed*)a.seeds, (const double*)a.seed_int,
+    hipLaunchKernelGGL(k_adj_seed, dim3(grid), dim3(64), 0, f->stream, f->env->dev, f->F, (const Slots*)a.ws->slots,
+                       (const double*)a.ws->times, (const AdjSeed*)a.seeds, (const double*)a.seed_int,
                        (const double*)a.seed_grad, n, D);
 #else
-    emu_check_launch(f->env->dev, f->F, a.W, nullptr);
+    emu_check_launch(f->env->dev, f->F, a.ws->W, nullptr);
     for (int64_t e = 0; e < n; ++e)
-        wq3d::adj::body_adj_seed(e, 0, f->env->dev, f->F, a.slots, a.times, a.seeds, a.seed_int, a.seed_grad, D);
+        wq3d::adj::body_adj_seed(e, 0, f->env->dev, f->F, a.ws->slots, a.ws->times, a.seeds, a.seed_int, a.seed_grad, D);
 #endif
 }
 
@@ -314,30 +373,30 @@ void launch_adj_seed(wq3d_fan* f, int64_t n) {
 wq3d_status sort_entries(wq3d_fan* f, int64_t n) {
     FanAdjoint& a = *f->adjoint;
     const int64_t N = f->F.N;
-    HIP_TRY(rt::memset_async(a.rbeg, 0, (size_t)N * sizeof(int64_t), f->stream));
-    HIP_TRY(rt::memset_async(a.rend, 0, (size_t)N * sizeof(int64_t), f->stream));
+    HIP_TRY(rt::memset_async(a.ws->rbeg, 0, (size_t)N * sizeof(int64_t), f->stream));
+    HIP_TRY(rt::memset_async(a.ws->rend, 0, (size_t)N * sizeof(int64_t), f->stream));
     if (n == 0) return WQ3D_OK;
 #ifndef WQ3D_EMULATE
-    size_t bytes = a.sort_bytes;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(a.sort_tmp, bytes, a.D.ent_key, a.skey, a.D.ent_val, a.sval, (int)n,
+    size_t bytes = a.ws->sort_bytes;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(a.ws->sort_tmp, bytes, a.ws->D.ent_key, a.ws->skey, a.ws->D.ent_val, a.ws->sval, (int)n,
                                                0, 64, f->stream));
     const unsigned grid = (unsigned)((n + kAdjBlock - 1) / kAdjBlock);
-    hipLaunchKernelGGL(k_adj_ranges, dim3(grid), dim3(kAdjBlock), 0, f->stream, (const unsigned long long*)a.skey, n,
-                       a.rbeg, a.rend);
+    hipLaunchKernelGGL(k_adj_ranges, dim3(grid), dim3(kAdjBlock), 0, f->stream, (const unsigned long long*)a.ws->skey, n,
+                       a.ws->rbeg, a.ws->rend);
 #else
     std::vector<std::pair<unsigned long long, double>> e((size_t)n);
-    for (int64_t k = 0; k < n; ++k) e[(size_t)k] = {a.D.ent_key[k], a.D.ent_val[k]};
+    for (int64_t k = 0; k < n; ++k) e[(size_t)k] = {a.ws->D.ent_key[k], a.ws->D.ent_val[k]};
     std::stable_sort(e.begin(), e.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
     const int shift = 8 + wq3d::adj::kEntryStepBits;
     for (int64_t k = 0; k < n; ++k) {
-        a.skey[k] = e[(size_t)k].first;
-        a.sval[k] = e[(size_t)k].second;
+        a.ws->skey[k] = e[(size_t)k].first;
+        a.ws->sval[k] = e[(size_t)k].second;
     }
     for (int64_t k = 0; k < n; ++k) {
-        const int64_t r = (int64_t)(a.skey[k] >> shift);
-        emu_range(a.rbeg + r, 1, "entry ray");
-        if (k == 0 || (int64_t)(a.skey[k - 1] >> shift) != r) a.rbeg[r] = k;
-        if (k == n - 1 || (int64_t)(a.skey[k + 1] >> shift) != r) a.rend[r] = k + 1;
+        const int64_t r = (int64_t)(a.ws->skey[k] >> shift);
+        emu_range(a.ws->rbeg + r, 1, "entry ray");
+        if (k == 0 || (int64_t)(a.ws->skey[k - 1] >> shift) != r) a.ws->rbeg[r] = k;
+        if (k == n - 1 || (int64_t)(a.ws->skey[k + 1] >> shift) != r) a.ws->rend[r] = k + 1;
     }
 #endif
     return WQ3D_OK;
@@ -346,23 +405,23 @@ wq3d_status sort_entries(wq3d_fan* f, int64_t n) {
 void launch_adj_sweep(wq3d_fan* f, int64_t K) {
     FanAdjoint& a = *f->adjoint;
     const FanConst& F = f->F;
-    AdjDev D = a.D;
+    AdjDev D = a.ws->D;
     D.tape_cap = kTapeSmall;
 #ifndef WQ3D_EMULATE
     const int64_t threads = std::min<int64_t>(tapes_of(kTapeSmall), (F.N + kAdjBlock - 1) / kAdjBlock * kAdjBlock);
     const unsigned grid = (unsigned)((threads + kAdjBlock - 1) / kAdjBlock);
     hipLaunchKernelGGL(k_adj_sweep, dim3(grid), dim3(kAdjBlock), a.n_nodes * sizeof(double), f->stream, f->env->dev,
-                       F, (const Slots*)a.slots, K, (const unsigned long long*)a.skey, (const double*)a.sval,
-                       (const int64_t*)a.rbeg, (const int64_t*)a.rend, D, (int32_t)a.n_nodes);
+                       F, (const Slots*)a.ws->slots, K, (const unsigned long long*)a.ws->skey, (const double*)a.ws->sval,
+                       (const int64_t*)a.ws->rbeg, (const int64_t*)a.ws->rend, D, (int32_t)a.n_nodes);
 #else
     auto add = [&](int k0, double d0, double d1) {
-        emu_range(a.D.node_grad + k0, 2, "node gradient");
-        a.D.node_grad[k0] += d0;
-        a.D.node_grad[k0 + 1] += d1;
+        emu_range(a.ws->D.node_grad + k0, 2, "node gradient");
+        a.ws->D.node_grad[k0] += d0;
+        a.ws->D.node_grad[k0 + 1] += d1;
     };
     for (int64_t k = 0; k < F.N; ++k) {
         const int64_t i = emu_ray(k, F.N);
-        wq3d::adj::body_adj_sweep(i, 0, f->env->dev, F, a.slots, K, a.skey, a.sval, a.rbeg[i], a.rend[i], D, add);
+        wq3d::adj::body_adj_sweep(i, 0, f->env->dev, F, a.ws->slots, K, a.ws->skey, a.ws->sval, a.ws->rbeg[i], a.ws->rend[i], D, add);
     }
 #endif
 }
@@ -370,7 +429,7 @@ void launch_adj_sweep(wq3d_fan* f, int64_t K) {
 void launch_adj_init(wq3d_fan* f, const Slots& S) {
     FanAdjoint& a = *f->adjoint;
     const FanConst& F = f->F;
-    AdjDev D = a.D;
+    AdjDev D = a.ws->D;
     D.tape_cap = kTapeSmall;
 #ifndef WQ3D_EMULATE
     const int64_t threads = std::min<int64_t>(tapes_of(kTapeSmall), (F.N + 63) & ~int64_t(63));
@@ -384,6 +443,7 @@ void launch_adj_init(wq3d_fan* f, const Slots& S) {
 }  // namespace
 
 void adjoint_free(wq3d_fan* f) {
+    if (f->adjoint) give_workspace(*f->adjoint);
     delete f->adjoint;
     f->adjoint = nullptr;
 }
@@ -467,7 +527,7 @@ wq3d_status wq3d_adjoint_gradient(wq3d_fan* f, const wq3d_eigenray* rays, const 
     }
     wq3d_status st = adjoint_workspace(f, longest + 1);
     if (st) return st;
-    AdjDev& D = a.D;
+    AdjDev& D = a.ws->D;
     HIP_TRY(rt::sync(f->stream));
     HIP_TRY(rt::memset(D.adj, 0, (size_t)(wq3d::adj::kAtt + nf) * 4 * F.N * sizeof(double)));
     HIP_TRY(rt::memset(D.node_grad, 0, (size_t)std::max<int64_t>(1, a.n_nodes) * sizeof(double)));
@@ -515,7 +575,7 @@ wq3d_status wq3d_adjoint_gradient(wq3d_fan* f, const wq3d_eigenray* rays, const 
         }
         next_ray = first;
         // 1. replay the segment into the history
-        st = copy_state(a.ws_mem, F, a.W, cp.A, f->stream, false);
+        st = copy_state(a.ws->mem, F, a.ws->W, cp.A, f->stream, false);
         if (st) return st;
         Replay rp{{cp.phys[0], cp.phys[1], cp.phys[2], cp.phys[3]}, cp.time, cp.steps};
         std::vector<Slots> slots_at((size_t)(K + 1));
@@ -524,19 +584,19 @@ wq3d_status wq3d_adjoint_gradient(wq3d_fan* f, const wq3d_eigenray* rays, const 
         times_at[0] = rp.time;
         launch_adj_capture(f, 0);
         for (int64_t h = 1; h <= K; ++h) {
-            launch_bounce(f->stream, grid, a.kpw, E, Fw, a.W, rp.slots(), a.Rw, rp.time, rp.steps);
+            launch_bounce(f->stream, grid, a.ws->kpw, E, Fw, a.ws->W, rp.slots(), a.ws->Rw, rp.time, rp.steps);
             rp.rotate();
             rp.time += F.dt;
             rp.steps += 1;
-            launch_advance(f->stream, grid, E, Fw, a.W, rp.slots());
+            launch_advance(f->stream, grid, E, Fw, a.ws->W, rp.slots());
             launch_adj_capture(f, h);
             slots_at[(size_t)h] = rp.slots();
             times_at[(size_t)h] = rp.time;
         }
         HIP_TRY(rt::last_error());
-        HIP_TRY(rt::memcpy_async(a.slots, slots_at.data(), slots_at.size() * sizeof(Slots), hipMemcpyHostToDevice,
+        HIP_TRY(rt::memcpy_async(a.ws->slots, slots_at.data(), slots_at.size() * sizeof(Slots), hipMemcpyHostToDevice,
                                  f->stream));
-        HIP_TRY(rt::memcpy_async(a.times, times_at.data(), times_at.size() * sizeof(double), hipMemcpyHostToDevice,
+        HIP_TRY(rt::memcpy_async(a.ws->times, times_at.data(), times_at.size() * sizeof(double), hipMemcpyHostToDevice,
                                  f->stream));
         // 2. the seeds of the segment's eigenrays, 3. sorted by ray and step
         HIP_TRY(rt::memset_async(D.ent_count, 0, sizeof(unsigned long long), f->stream));
@@ -565,6 +625,7 @@ wq3d_status wq3d_adjoint_gradient(wq3d_fan* f, const wq3d_eigenray* rays, const 
         return fail(WQ3D_ERR_STATE, "the replay does not reproduce the forward run or its eigenrays (code " +
                                         std::to_string(err) + ")");
     HIP_TRY(rt::memcpy(grad_nodes, D.node_grad, (size_t)a.n_nodes * sizeof(double), hipMemcpyDeviceToHost));
+    give_workspace(a);  // the next pass, of this fan or another, takes it from the pool
     return WQ3D_OK;
 }
 

@@ -88,18 +88,27 @@ class profile_runs:
     def counts(self):
         return [len(r) for r, _ in self.records]
 
-    def gradient(self, grad_time, grad_intensity):
+    def gradient(self, grad_time, grad_intensity, workers=4, release=False):
         """d loss / d node speeds [P, n] from the gradients with respect to
-        every run's eigenrays (concatenated in run order)."""
+        every run's eigenrays (concatenated in run order).  `workers` fans
+        run their backward passes at once (each on its own stream; about 12
+        GB of device workspace each at config 5's size).  release: free each
+        fan's checkpoints and workspace after its pass (no second backward)."""
+        from concurrent.futures import ThreadPoolExecutor
         grad_time = np.asarray(grad_time, dtype=np.float64).reshape(-1)
         grad_intensity = np.asarray(grad_intensity, dtype=np.float64).reshape(-1, self.nf)
         out = np.zeros((len(self.queues), self.n_nodes))
-        at = 0
-        for p, (q, (recs, inten)) in enumerate(zip(self.queues, self.records)):
-            n = len(recs)
-            out[p] = q.adjoint_gradient(recs, inten, grad_time[at:at + n], grad_intensity[at:at + n],
-                                        self.n_nodes)
-            at += n
+        starts = np.concatenate([[0], np.cumsum(self.counts())]).astype(int)
+
+        def one(p):
+            q, (recs, inten) = self.queues[p], self.records[p]
+            a, b = starts[p], starts[p + 1]
+            out[p] = q.adjoint_gradient(recs, inten, grad_time[a:b], grad_intensity[a:b], self.n_nodes)
+            if release:
+                q.adjoint_release()
+
+        with ThreadPoolExecutor(max_workers=max(1, int(workers))) as pool:
+            list(pool.map(one, range(len(self.queues))))
         return out
 
 
@@ -126,5 +135,5 @@ class eigenray_function(torch.autograd.Function):
             gt = np.zeros(n)
         if np.isscalar(gi):
             gi = np.zeros((n, ctx.runs.nf))
-        g = ctx.runs.gradient(gt, gi)
+        g = ctx.runs.gradient(gt, gi, release=True)
         return torch.from_numpy(g).to(ctx.device, ctx.dtype), None
