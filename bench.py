@@ -352,6 +352,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--legs", default="config1,config2fan,collection,sources,adjoint")
     ap.add_argument("--adjoint-workers", type=int, default=4)
+    ap.add_argument("--adjoint-profiles", type=int, default=None,
+                    help="profiles in the adjoint leg (default: --profiles)")
     ap.add_argument("--sources", type=int, default=64, help="config 4 sources in the batch leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -437,7 +439,7 @@ def main():
         legs["config4_sources"] = leg_sources(lib, args.sources)
         log(f"[bench] config 4 sources: {legs['config4_sources']['speedup']:.2f}x batched over alone")
     if "adjoint" in names:
-        legs["adjoint"] = leg_adjoint(lib, n, args.adjoint_workers)
+        legs["adjoint"] = leg_adjoint(lib, args.adjoint_profiles or n, args.adjoint_workers)
         log(f"[bench] adjoint: forward {legs['adjoint']['forward_batched_s']:.2f} s, backward "
             f"{legs['adjoint']['backward_s']:.2f} s; one profile {legs['adjoint']['one_profile']['backward_over_forward']:.2f}x")
     out["legs"] = legs

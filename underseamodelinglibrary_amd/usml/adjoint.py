@@ -92,8 +92,10 @@ class profile_runs:
         """d loss / d node speeds [P, n] from the gradients with respect to
         every run's eigenrays (concatenated in run order).  `workers` fans
         run their backward passes at once (each on its own stream; about 12
-        GB of device workspace each at config 5's size).  release: free each
-        fan's checkpoints and workspace after its pass (no second backward)."""
+        GB of device workspace each at config 5's size, pooled between
+        passes).  release: free every fan's checkpoints once all passes are
+        done (no second backward); freeing device memory synchronises the
+        device, so it waits until no pass is running."""
         from concurrent.futures import ThreadPoolExecutor
         grad_time = np.asarray(grad_time, dtype=np.float64).reshape(-1)
         grad_intensity = np.asarray(grad_intensity, dtype=np.float64).reshape(-1, self.nf)
@@ -104,11 +106,12 @@ class profile_runs:
             q, (recs, inten) = self.queues[p], self.records[p]
             a, b = starts[p], starts[p + 1]
             out[p] = q.adjoint_gradient(recs, inten, grad_time[a:b], grad_intensity[a:b], self.n_nodes)
-            if release:
-                q.adjoint_release()
 
         with ThreadPoolExecutor(max_workers=max(1, int(workers))) as pool:
             list(pool.map(one, range(len(self.queues))))
+        if release:
+            for q in self.queues:
+                q.adjoint_release()
         return out
 
 
