@@ -180,20 +180,26 @@ void for_each_field(const FanConst& F, Arrays& dst, const Arrays& src, Fn fn) {
     fn((void**)&dst.flag, (const void*)src.flag, F.N * sizeof(uint8_t));
 }
 
-// allocate dst's fields in mem (when alloc) and copy src into them on the stream
+// allocate dst's fields in mem (when alloc: one allocation, 256-byte aligned
+// fields, so that freeing a copy is one hipFree) and copy src into them on
+// the stream
 wq3d_status copy_state(DeviceBuffers& mem, const FanConst& F, Arrays& dst, const Arrays& src, hipStream_t st,
                        bool alloc) {
     Arrays s = src;
     bool ok = true;
+    uint8_t* base = nullptr;
+    if (alloc) {
+        size_t total = 0;
+        Arrays probe{};
+        for_each_field(F, probe, s, [&](void**, const void*, size_t bytes) { total += (bytes + 255) & ~size_t(255); });
+        if (mem.alloc(&base, total) != hipSuccess) return fail(WQ3D_ERR_DEVICE, "adjoint state allocation failed");
+    }
+    size_t at = 0;
     for_each_field(F, dst, s, [&](void** d, const void* p, size_t bytes) {
         if (!ok) return;
         if (alloc) {
-            uint8_t* q = nullptr;
-            if (mem.alloc(&q, bytes) != hipSuccess) {
-                ok = false;
-                return;
-            }
-            *d = q;
+            *d = base + at;
+            at += (bytes + 255) & ~size_t(255);
         }
         if (rt::memcpy_async(*d, p, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) ok = false;
     });
