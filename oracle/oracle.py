@@ -46,6 +46,8 @@ def load():
         getattr(lib, n).restype = i64
         getattr(lib, n).argtypes = [vp]
     lib.orc_take_eigenrays.argtypes = [vp, C.POINTER(_abi.EigenrayRec), dp, dp]
+    lib.orc_sign_rule_counts.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
+    lib.orc_sign_rule_counts.restype = None
     lib.orc_take_eigenverbs.argtypes = [vp, C.POINTER(_abi.EigenverbRec), dp]
     lib.orc_take_reflections.argtypes = [vp, C.POINTER(_abi.ReflectionRec)]
     lib.orc_snapshot.argtypes = [vp, C.c_int32, C.POINTER(_abi.FrontView)]

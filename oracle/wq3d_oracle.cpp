@@ -48,8 +48,10 @@
  *   Gaussian walk's stopping ratio (1.002305238) - moving it to 1.1 changes
  *   the pinned scenarios by less than their tolerances - the fold test that
  *   counts caustics (pinned only through the phase it adds, caustics
- *   occurring in the Munk channel and none on Lloyd's mirror paths) and the
- *   geometric DE sign on compute_offsets' unstable branch.
+ *   occurring in the Munk channel and none on Lloyd's mirror paths).  The
+ *   geometric DE sign on compute_offsets' unstable branch is pinned by
+ *   straight rays: every direct eigenray's DE offset points toward the chord
+ *   to its target.
  */
 #include <algorithm>
 #include <cmath>
@@ -1400,6 +1402,7 @@ struct Queue {
         if (offsets_core(D, delta, unstable, offset, distance) && distance[1] > 0.0) {
             // the reference's else-branch (a positive DE remainder) takes its
             // sign from the direction toward the next DE ray and the target
+            ++sign_rule_count;
             double cpa[3], nxt[3], prv[3], dir[3], speed;
             collision_location(de, az, offset[0], cpa, dir, &speed);
             collision_location(de + 1, az, offset[0], nxt, dir, &speed);
@@ -1416,9 +1419,13 @@ struct Queue {
             if (dot < 0.0) {
                 distance[1] *= -1.0;
                 offset[1] *= -1.0;
+                ++sign_flip_count;
             }
         }
     }
+    // how often compute_offsets' unstable branch took the geometric DE sign,
+    // and how often that sign was negative (a test hook)
+    int64_t sign_rule_count = 0, sign_flip_count = 0;
 
     // compute_offsets (wave_queue.cc:708-924) from the neighbourhood's
     // distances alone: make_taylor_coeff, the stability tests (determinant
@@ -1940,6 +1947,14 @@ void orc_source(void* h, double* r, double* t, double* p) {
 }
 
 int64_t orc_num_eigenrays(void* h) { return (int64_t) static_cast<orc_handle*>(h)->q.rays.size(); }
+
+/* Test hook: the number of eigenrays whose DE offset took its sign from the
+ * geometry (compute_offsets' unstable branch) and how many of those signs
+ * were negative. */
+void orc_sign_rule_counts(void* h, int64_t* taken, int64_t* negative) {
+    *taken = static_cast<orc_handle*>(h)->q.sign_rule_count;
+    *negative = static_cast<orc_handle*>(h)->q.sign_flip_count;
+}
 int64_t orc_num_eigenverbs(void* h) { return (int64_t) static_cast<orc_handle*>(h)->q.verbs.size(); }
 int64_t orc_num_reflections(void* h) { return (int64_t) static_cast<orc_handle*>(h)->q.refl.size(); }
 

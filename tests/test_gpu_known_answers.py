@@ -78,3 +78,22 @@ def test_refraction_munk_range_flat_earth_gpu():
 def test_reflection_reinit_image_source_gpu():
     print("[known answer] image-source chains: 21 rays, 120 steps", flush=True)
     pins.image_source_check(lambda *a: wave_queue(*a))
+
+
+@pytest.mark.gpu
+def test_de_offset_sign_straight_rays_gpu(oracle_mod):
+    """The straight-ray DE sign check of test_oracle_pins on the device's
+    eigenrays, which also equal the oracle's."""
+    from test_oracle_pins import check_de_offsets_point_to_targets
+    from parity import compare_eigenrays
+    from underseamodelinglibrary_amd import scenarios
+    from underseamodelinglibrary_amd.usml.waveq3d import wave_queue
+    sc = scenarios.straight_ray_targets()
+    g = wave_queue(*sc.args())
+    o = oracle_mod.oracle_queue(*sc.args())
+    g.step(sc.steps)
+    o.step(sc.steps)
+    gr, gi, gp = g.drain_eigenrays()
+    orr, oi, op = o.drain_eigenrays()
+    compare_eigenrays(gr, gi, gp, orr, oi, op, sc.freq.size())
+    assert check_de_offsets_point_to_targets(sc, gr) > 100

@@ -244,6 +244,50 @@ def config2_analytic(n_targets=(100, 100), time_max=120.0):
     return sc
 
 
+def straight_ray_targets():
+    """Straight rays for the sign of compute_offsets' DE offset
+    (wave_queue.cc:863-912): an isovelocity ocean (1500 m/s, no absorption)
+    with its boundaries 10 km above and below the source (45N 45W, 10 km
+    deep), a coarse fan (DE -60:5:60, AZ 0:15:360 closing on itself), 1 kHz,
+    dt 0.1 s for 6 s, and 12 x 12 seeded targets within 3 km of the source's
+    depth and 0.05 deg of its position.  Direct paths only; each target's
+    launch elevation is that of the straight chord from the source."""
+    compute_earth_radius(45.0)
+    profile = oc.profile_linear(1500.0, oc.attenuation_constant(0.0))
+    ocean = oc.ocean_model(oc.boundary_flat(), oc.boundary_flat(20000.0), profile)
+    rng = np.random.default_rng(3)
+    n1 = n2 = 12
+    lat = np.zeros((n1, n2))
+    lng = np.zeros((n1, n2))
+    alt = np.zeros((n1, n2))
+    for a in range(n1):
+        for b in range(n2):
+            lat[a, b] = 45.0 + rng.uniform(-0.04, 0.04)
+            lng[a, b] = -45.0 + rng.uniform(-0.05, 0.05)
+            alt[a, b] = -10000.0 + rng.uniform(-3000.0, 3000.0)
+    sc = scenario("straight_ray_targets", ocean, seq_log(1000.0, 1.0, 1), wposition1(45.0, -45.0, -10000.0),
+                  seq_linear(-60.0, 5.0, 60.0), seq_linear(0.0, 15.0, 360.0), 0.1,
+                  wposition.from_arrays(lat, lng, alt), loop_steps(0.1, 6.0))
+    sc.notes["targets"] = (lat, lng, alt)
+    return sc
+
+
+def chord_elevation_deg(source, lat, lng, alt):
+    """Elevation (deg, up positive) of the straight chord from `source`
+    (a wposition1) to the point (lat, lng, alt) on the current earth
+    sphere: the launch DE of a straight ray that reaches it."""
+    R = earth_radius()
+
+    def xyz(rho, lat_deg, lng_deg):
+        th = math.radians(90.0 - lat_deg)
+        ph = math.radians(lng_deg)
+        return np.array([rho * math.sin(th) * math.cos(ph), rho * math.sin(th) * math.sin(ph), rho * math.cos(th)])
+
+    s = xyz(R + source.altitude(), source.latitude(), source.longitude())
+    d = xyz(R + alt, lat, lng) - s
+    return math.degrees(math.asin(float(np.dot(d, s)) / (np.linalg.norm(d) * np.linalg.norm(s))))
+
+
 def eigenray_branch_pt():
     """waveq3d/test/eigenray_test.cc:428-534 scenario: AZ fan closing on
     itself, DE -90:1:90, 12 targets on a 2226 m ring plus targets 500 m
