@@ -353,7 +353,7 @@ def main():
     ap.add_argument("--adjoint-workers", type=int, default=4)
     ap.add_argument("--adjoint-profiles", type=int, default=None,
                     help="profiles in the adjoint leg (default: --profiles)")
-    ap.add_argument("--sources", type=int, default=64, help="config 4 sources in the batch leg")
+    ap.add_argument("--sources", type=int, default=256, help="config 4 sources in the batch leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
