@@ -48,6 +48,10 @@ def load():
     lib.orc_take_eigenrays.argtypes = [vp, C.POINTER(_abi.EigenrayRec), dp, dp]
     lib.orc_sign_rule_counts.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     lib.orc_sign_rule_counts.restype = None
+    lib.orc_trace_walks.argtypes = [vp, C.c_int32]
+    lib.orc_trace_walks.restype = None
+    lib.orc_walk_trace.argtypes = [vp, dp, i64]
+    lib.orc_walk_trace.restype = i64
     lib.orc_take_eigenverbs.argtypes = [vp, C.POINTER(_abi.EigenverbRec), dp]
     lib.orc_take_reflections.argtypes = [vp, C.POINTER(_abi.ReflectionRec)]
     lib.orc_snapshot.argtypes = [vp, C.c_int32, C.POINTER(_abi.FrontView)]

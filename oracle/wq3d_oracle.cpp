@@ -44,14 +44,13 @@
  *   Rayleigh loss against the textbook reflection coefficient; the shared
  *   elementary functions against glibc in ulps; the eigenray collection's
  *   191.4 dB total (eigenrays/test/eigenrays_test.cc:57-88) and closed
- *   forms of the coherent / incoherent sums.  Parity unpinned: the hybrid
- *   Gaussian walk's stopping ratio (1.002305238) - moving it to 1.1 changes
- *   the pinned scenarios by less than their tolerances - the fold test that
- *   counts caustics (pinned only through the phase it adds, caustics
- *   occurring in the Munk channel and none on Lloyd's mirror paths).  The
- *   geometric DE sign on compute_offsets' unstable branch is pinned by
- *   straight rays: every direct eigenray's DE offset points toward the chord
- *   to its target.
+ *   forms of the coherent / incoherent sums; the hybrid Gaussian walks'
+ *   stopping rule against the documented 0.01 dB (10^(0.001)) on traced walk
+ *   steps; the caustic fold test against ray theory (caustics = turning
+ *   points, or one fewer, for refracted rays in the Munk channel); the
+ *   geometric DE sign on compute_offsets' unstable branch by straight rays
+ *   (every direct eigenray's DE offset points toward the chord to its
+ *   target).  No function is left unpinned.
  */
 #include <algorithm>
 #include <cmath>
@@ -574,6 +573,15 @@ struct Spreading {
     double init_c = 0.0;      // spreading_ray::_init_sound_speed
     static constexpr double OVERLAP = 2.0;
     static constexpr double THRESHOLD = 1.002305238;
+    // test hook: when on, every walk step appends (walk 0-3, sum before,
+    // sum after, 0 going on / 1 stopped by THRESHOLD / 2 stopped at a
+    // virtual ray) for the lowest frequency
+    bool trace_walks = false;
+    std::vector<double> walk_trace;
+    void trace(int walk, double before, double after, int how) {
+        if (!trace_walks) return;
+        walk_trace.insert(walk_trace.end(), {(double)walk, before, after, (double)how});
+    }
     void init(Queue* q);
     void gaussian(double d, double wdt, double A, double* out);
     void intensity(double trho, double ttheta, double tphi, size_t de, size_t az,
@@ -1727,7 +1735,11 @@ void Spreading::intensity_de(size_t de, size_t az, const double offset[3],
         const double old_tl = I_de[0];
         gaussian(cell_dist, cell_width, nn, g.data());
         for (size_t f = 0; f < nf; ++f) I_de[f] += g[f];
-        if (I_de[0] / old_tl < THRESHOLD) break;
+        if (I_de[0] / old_tl < THRESHOLD) {
+            trace(0, old_tl, I_de[0], 1);
+            break;
+        }
+        trace(0, old_tl, I_de[0], virtual_ray ? 2 : 0);
         if (virtual_ray) break;
     }
     cell_width = initial_width;
@@ -1748,7 +1760,11 @@ void Spreading::intensity_de(size_t de, size_t az, const double offset[3],
         const double old_tl = I_de[0];
         gaussian(cell_dist, cell_width, nn, g.data());
         for (size_t f = 0; f < nf; ++f) I_de[f] += g[f];
-        if (I_de[0] / old_tl < THRESHOLD) break;
+        if (I_de[0] / old_tl < THRESHOLD) {
+            trace(1, old_tl, I_de[0], 1);
+            break;
+        }
+        trace(1, old_tl, I_de[0], virtual_ray ? 2 : 0);
         if (virtual_ray) break;
     }
 }
@@ -1802,7 +1818,11 @@ void Spreading::intensity_az(size_t de, size_t az, const double offset[3],
         const double old_tl = I_az[0];
         gaussian(cell_dist, cell_width, norm(de, a), g.data());
         for (size_t f = 0; f < nf; ++f) I_az[f] += g[f];
-        if (I_az[0] / old_tl < THRESHOLD) break;
+        if (I_az[0] / old_tl < THRESHOLD) {
+            trace(2, old_tl, I_az[0], 1);
+            break;
+        }
+        trace(2, old_tl, I_az[0], 0);
         if (a == 0) {
             a += max_az - 1;
         } else {
@@ -1823,7 +1843,11 @@ void Spreading::intensity_az(size_t de, size_t az, const double offset[3],
         const double old_tl = I_az[0];
         gaussian(cell_dist, cell_width, norm(de, a), g.data());
         for (size_t f = 0; f < nf; ++f) I_az[f] += g[f];
-        if (I_az[0] / old_tl < THRESHOLD) break;
+        if (I_az[0] / old_tl < THRESHOLD) {
+            trace(3, old_tl, I_az[0], 1);
+            break;
+        }
+        trace(3, old_tl, I_az[0], 0);
         ++a;
     }
 }
@@ -1947,6 +1971,21 @@ void orc_source(void* h, double* r, double* t, double* p) {
 }
 
 int64_t orc_num_eigenrays(void* h) { return (int64_t) static_cast<orc_handle*>(h)->q.rays.size(); }
+
+/* Test hook: trace the hybrid Gaussian walks (on / off), and copy out the
+ * trace (4 doubles per walk step; returns the number of doubles, copying at
+ * most cap of them). */
+void orc_trace_walks(void* h, int32_t on) {
+    auto& sp = static_cast<orc_handle*>(h)->q.spreading;
+    sp.trace_walks = on != 0;
+    sp.walk_trace.clear();
+}
+int64_t orc_walk_trace(void* h, double* out, int64_t cap) {
+    const auto& t = static_cast<orc_handle*>(h)->q.spreading.walk_trace;
+    const int64_t n = (int64_t)t.size();
+    for (int64_t k = 0; k < n && k < cap; ++k) out[k] = t[(size_t)k];
+    return n;
+}
 
 /* Test hook: the number of eigenrays whose DE offset took its sign from the
  * geometry (compute_offsets' unstable branch) and how many of those signs

@@ -97,3 +97,24 @@ def test_de_offset_sign_straight_rays_gpu(oracle_mod):
     orr, oi, op = o.drain_eigenrays()
     compare_eigenrays(gr, gi, gp, orr, oi, op, sc.freq.size())
     assert check_de_offsets_point_to_targets(sc, gr) > 100
+
+
+@pytest.mark.gpu
+def test_caustic_fold_test_matches_turning_points_gpu(oracle_mod):
+    """The caustic / turning-point relation of test_oracle_pins on the
+    device's wavefront, whose counters also equal the oracle's."""
+    import numpy as np
+    from test_oracle_pins import check_caustics_are_turning_points
+    from underseamodelinglibrary_amd import _abi
+    from underseamodelinglibrary_amd.usml.types import seq_linear
+    from underseamodelinglibrary_amd.usml.waveq3d import wave_queue
+    sc = scenarios.config1(de=seq_linear(-20.0, 0.25, 20.0), az=seq_linear(0.0, 90.0, 270.0), targets=False,
+                           time_max=120.0)
+    g = wave_queue(*sc.args())
+    o = oracle_mod.oracle_queue(*sc.args())
+    g.step(sc.steps)
+    o.step(sc.steps)
+    gf, of = g.snapshot_arrays(_abi.CURR), o.snapshot_arrays(_abi.CURR)
+    for k in ("surface", "bottom", "caustic", "upper", "lower"):
+        assert np.array_equal(np.asarray(gf[k]), np.asarray(of[k])), k
+    assert check_caustics_are_turning_points(sc, gf) > 400

@@ -506,3 +506,66 @@ def test_unstable_branch_de_sign_points_to_target(oracle_mod):
     assert taken.value >= 10 and 0 < negative.value < taken.value, (taken.value, negative.value)
     assert len(recs) > 100
     assert check_de_offsets_point_to_targets(sc, recs) > 100
+
+
+def check_caustics_are_turning_points(sc, front):
+    """WKB: in a refracting channel every caustic a ray meets is one of its
+    turning points (each costs the -pi/2 the caustic counter adds).  For the
+    rays that touch neither boundary, and whose DE neighbours do not either
+    (the fold test only pairs rays with the same boundary counts), the
+    caustic count is the number of upper and lower vertices, or one less:
+    the first turning point comes before the ray family folds.  Returns the
+    number of rays checked."""
+    import numpy as np
+    naz = sc.az.size()
+    srf, btm, cst, upr, lwr = (np.asarray(front[k]).reshape(-1, naz)
+                               for k in ("surface", "bottom", "caustic", "upper", "lower"))
+    free = (srf == 0) & (btm == 0)
+    inner = free[1:-1] & free[:-2] & free[2:]
+    turns = (upr + lwr)[1:-1][inner]
+    caustics = cst[1:-1][inner]
+    assert (turns >= 2).all()
+    assert ((caustics == turns) | (caustics == turns - 1)).all(), np.unique(caustics - turns, return_counts=True)
+    return int(inner.sum())
+
+
+def test_caustic_fold_test_matches_turning_points(oracle_mod):
+    """The fold test that counts caustics (wave_queue.cc:274-292) against
+    ray theory in config 1's Munk channel: 120 s of DE -20:0.25:20.  An
+    inverted crossing test, or a fold test that pairs rays whose boundary
+    counts differ, breaks the relation."""
+    from underseamodelinglibrary_amd import _abi, scenarios
+    from underseamodelinglibrary_amd.usml.types import seq_linear
+    sc = scenarios.config1(de=seq_linear(-20.0, 0.25, 20.0), az=seq_linear(0.0, 90.0, 270.0), targets=False,
+                           time_max=120.0)
+    o = oracle_mod.oracle_queue(*sc.args())
+    o.step(sc.steps)
+    assert check_caustics_are_turning_points(sc, o.snapshot_arrays(_abi.CURR)) > 400
+
+
+def test_gaussian_walk_stops_at_one_hundredth_of_a_db(oracle_mod):
+    """The hybrid Gaussian walks' stopping rule (spreading_hybrid_gaussian.h:
+    "iteration stops when new contribution makes less than a 0.01 dB
+    contribution to the overall result"): in every traced walk step, the
+    step stops the walk by the ratio test exactly when its cell raised the
+    lowest frequency's sum by less than 10^(0.01/10) - the documented 0.01 dB,
+    derived here, not the oracle's constant - and never stops it otherwise
+    (a virtual ray ends a DE walk separately).  Config 1's Munk ocean with
+    its 100 targets, 600 steps: thousands of walk steps of both kinds."""
+    import ctypes as C
+    import numpy as np
+    from underseamodelinglibrary_amd import scenarios
+    sc = scenarios.config1()
+    o = oracle_mod.oracle_queue(*sc.args())
+    o._lib.orc_trace_walks(o._h, 1)
+    o.step(sc.steps)
+    n = o._lib.orc_walk_trace(o._h, None, 0)
+    buf = np.zeros(n)
+    o._lib.orc_walk_trace(o._h, buf.ctypes.data_as(C.POINTER(C.c_double)), n)
+    walk, before, after, how = buf.reshape(-1, 4).T
+    limit = 10.0 ** (0.01 / 10.0)
+    below = after / before < limit
+    assert len(walk) > 10000 and below.sum() > 1000 and (~below).sum() > 1000
+    assert (how[below] == 1).all()
+    assert (how[~below] != 1).all()
+    assert set(np.unique(walk).astype(int)) == {0, 1, 2, 3}
