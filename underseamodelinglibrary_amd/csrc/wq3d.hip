@@ -65,6 +65,41 @@ constexpr int kHitQueue = 128;
 // MI355X accelerator dies (XCDs), each with its own L2; consecutive
 // workgroups of a launch are dispatched to them in turn
 constexpr int kXcds = 8;
+// Reflection lists of a fused batch step.  k_reflect_light_multi lists the
+// rays k_bounce_multi takes (flag[] != 0, or every ray when volume layers
+// emit eigenverbs), so that the reflection launch does not dispatch a
+// wavefront for each of the many rays that do not reflect.  The lists stay
+// on the die of the rays' own workgroup (workgroup w of a launch runs on
+// die w % kXcds, and the propagation launches map rays to workgroups the
+// same way, so a ray's state is in that die's L2): die x's list holds the
+// listed rays of the listing launch's workgroups x, x + kXcds, ... in
+// those workgroups' slots of the list array, entry j at slot
+// (x + (j / threads) * kXcds) * threads + j % threads.  The reflection
+// launch has one wavefront per workgroup, so that a die's listed rays
+// spread over its CUs (bounce_grid); its workgroup w takes entries
+// (w / kXcds) * 64 + lane of die w % kXcds's list.  Each die has two lists,
+// surface reflections from the front of its slots and bottom reflections
+// from the back, so that a wavefront does not run both kinds one after the
+// other; their lengths are counted by step parity, [parity][die][front,
+// back], and the reflection launch clears the other parity's.
+constexpr int kListCounts = 2 * kXcds * 2;
+// workgroups of a grid of `blocks` that run on die x
+WQ_HD_INLINE int64_t die_blocks(int64_t blocks, int x) { return x < blocks ? (blocks - 1 - x) / kXcds + 1 : 0; }
+// slot of entry j of die x's list, for `threads` per workgroup
+WQ_HD_INLINE int64_t die_slot(int x, int64_t j, int64_t threads) {
+    return (x + (j / threads) * kXcds) * threads + j % threads;
+}
+// the reflection launch's workgroups of 64 threads for a listing launch of
+// `blocks` x `threads`: on each die, as many threads as its lists can hold
+inline unsigned bounce_grid(int64_t blocks, int64_t threads) {
+    return (unsigned)(kXcds * ((blocks + kXcds - 1) / kXcds) * (threads / 64));
+}
+// entry k of die x's lists (front entries, then the back list from the end),
+// for the lists' lengths c[0], c[1] and `cap` slots of the die
+WQ_HD_INLINE int64_t die_entry(int64_t k, const unsigned long long* c, int64_t cap) {
+    return k < (int64_t)c[0] ? k : cap - 1 - (k - (int64_t)c[0]);
+}
+
 static_assert(kTargetBlock == 64, "k_detect tests one target per lane");
 
 // The parameters of a fan in device memory, for kernels whose per-ray body
@@ -461,10 +496,29 @@ __device__ __forceinline__ int fan_of_block(const int64_t* __restrict__ first, i
     return lo;
 }
 
+// Appends this wavefront's lanes with `want` to a list: one atomic per
+// wavefront, slots in lane order; write(k) stores the lane's entry at k.
+template <class Write>
+__device__ __forceinline__ void wave_append(bool want, unsigned long long* count, Write write) {
+    const unsigned long long m = __ballot(want);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    if (want) write(base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull)));
+}
+
+// The group's reflection lists (above) hold (fan of the group << 32 | ray);
+// the step parity is the fans' common step count before the rotation, read
+// from the device clocks so that a replayed graph reads the current one.
 __global__ void __launch_bounds__(kBlock) k_reflect_light_multi(const KParams* const* __restrict__ kps,
                                                                const int32_t* __restrict__ ids,
                                                                const int64_t* __restrict__ first, int n,
-                                                               Slots S) {
+                                                               Slots S, const FanClock* __restrict__ clk,
+                                                               unsigned long long* __restrict__ blist,
+                                                               unsigned long long* __restrict__ bcnt) {
     const int g = fan_of_block(first, n, blockIdx.x);
     const KParams* P = kps[ids[g]];
     const int64_t i = (int64_t)(blockIdx.x - first[g]) * blockDim.x + threadIdx.x;
@@ -473,22 +527,44 @@ __global__ void __launch_bounds__(kBlock) k_reflect_light_multi(const KParams* c
         *P->R.hit_count = 0ull;
         *P->R.pass_count = 0ull;
     }
-    if (i >= P->F.N) return;
-    body_reflect_light(i, P->F, P->A, S);
+    const bool ray = i < P->F.N;
+    if (ray) body_reflect_light(i, P->F, P->A, S);
+    const int parity = (int)(clk[ids[0]].step & 1);
+    const bool want = ray && bounce_wanted(P->E, P->F, P->A, i);
+    const int side = want ? bounce_side(P->A, i) : 0;
+    const unsigned long long e = ((unsigned long long)g << 32) | (unsigned long long)i;
+    const int x = (int)(blockIdx.x % kXcds);
+    const int64_t tpb = blockDim.x, cap = die_blocks(gridDim.x, x) * tpb;
+    unsigned long long* cnt = bcnt + (parity * kXcds + x) * 2;
+    wave_append(want && side == 0, cnt, [&](unsigned long long k) { blist[die_slot(x, (int64_t)k, tpb)] = e; });
+    wave_append(want && side == 1, cnt + 1,
+                [&](unsigned long long k) { blist[die_slot(x, cap - 1 - (int64_t)k, tpb)] = e; });
 }
 
+// k_bounce over the group's lists: one listed ray per thread on
+// bounce_grid(lblocks, lthreads), the listing launch's grid and workgroup
+// size; the listed rays fill the first wavefronts of each die, the rest
+// leave after reading the lengths (a grid-stride loop instead doubles the
+// kernel's spills)
 template <int PK, bool FLAT>
-__global__ void __launch_bounds__(kBlock) k_bounce_multi(const KParams* const* __restrict__ kps,
-                                                        const int32_t* __restrict__ ids,
-                                                        const int64_t* __restrict__ first, int n, Slots S,
-                                                        const FanClock* __restrict__ clk) {
-    const int g = fan_of_block(first, n, blockIdx.x);
-    const int f = ids[g];
+__global__ void __launch_bounds__(64) k_bounce_multi(const KParams* const* __restrict__ kps,
+                                                    const int32_t* __restrict__ ids, Slots S,
+                                                    const FanClock* __restrict__ clk,
+                                                    const unsigned long long* __restrict__ blist,
+                                                    unsigned long long* __restrict__ bcnt, int64_t lblocks,
+                                                    int64_t lthreads) {
+    const int parity = (int)(clk[ids[0]].step & 1);
+    if (blockIdx.x == 0 && threadIdx.x < 2 * kXcds) bcnt[(parity ^ 1) * 2 * kXcds + threadIdx.x] = 0ull;
+    const int x = (int)(blockIdx.x % kXcds);
+    const unsigned long long* c = bcnt + (parity * kXcds + x) * 2;
+    const int64_t j = (int64_t)(blockIdx.x / kXcds) * 64 + threadIdx.x;
+    if (j >= (int64_t)(c[0] + c[1])) return;
+    const int64_t k = die_entry(j, c, die_blocks(lblocks, x) * lthreads);
+    const unsigned long long e = blist[die_slot(x, k, lthreads)];
+    const int f = ids[(int)(e >> 32)];
     const KParams* P = kps[f];
-    const int64_t i = (int64_t)(blockIdx.x - first[g]) * blockDim.x + threadIdx.x;
-    if (i >= P->F.N) return;
-    if (P->A.flag[i] == 0 && !(P->F.want_verbs && P->E.nvol > 0)) return;
-    body_bounce<KindsOf<PK, FLAT>>(i, P->E, P->F, P->A, S, P->R, clk[f].time, clk[f].step);
+    body_bounce<KindsOf<PK, FLAT>>((int64_t)(e & 0xffffffffull), P->E, P->F, P->A, S, P->R, clk[f].time,
+                                   clk[f].step);
 }
 
 template <int PK, bool FLAT>

@@ -89,6 +89,12 @@ struct wq3d_batch {
         int32_t* d_ids = nullptr;
         int64_t* d_first = nullptr;
         int64_t blocks = 0;
+        int64_t rays = 0;                       // over the group's fans
+        unsigned long long* d_blist = nullptr;  // [slots] this step's listed rays, (fan of the group << 32 | ray),
+                                                //   in lists per die as k_reflect_light's
+        unsigned long long* d_bcnt = nullptr;   // [2][dies][2] the lists' lengths by step parity
+        int64_t slots = 0;                      // rays + 256 per fan: the grid's threads at any ray_threads
+        std::vector<int64_t> first;             // host copy of d_first
     };
     std::vector<Group> groups;
     std::vector<int32_t> tids;  // fans with targets (the eigenray search)
@@ -125,6 +131,7 @@ wq3d_status batch_tables(wq3d_batch* b) {
             first[k + 1] = first[k] + (b->fans[g.ids[k]]->F.N + tpb - 1) / tpb;
         g.blocks = first.back();
         HIP_TRY(rt::memcpy(g.d_first, first.data(), first.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+        g.first = first;
     }
     return WQ3D_OK;
 }
@@ -137,20 +144,24 @@ wq3d_status batch_launch_step(wq3d_batch* b, Launcher& L, const Slots& S, int rs
     const int n = (int)b->fans.size();
     const KParams* const* kps = b->kps + (size_t)rset * n;
     const dim3 tpb((unsigned)b->shape.ray_threads);
+    const FanClock* clk = b->clk;
     for (const auto& g : b->groups)
-        L(&k_reflect_light_multi, dim3((unsigned)g.blocks), tpb, kps, g.d_ids, g.d_first, (int)g.ids.size(), S);
+        L(&k_reflect_light_multi, dim3((unsigned)g.blocks), tpb, kps, g.d_ids, g.d_first, (int)g.ids.size(), S, clk,
+          g.d_blist, g.d_bcnt);
     for (const auto& g : b->groups) {
-        const int ng = (int)g.ids.size();
-        const dim3 grid((unsigned)g.blocks);
+        const unsigned long long* blist = g.d_blist;
+        const int64_t lb = g.blocks, lt = b->shape.ray_threads;
+        const dim3 grid(bounce_grid(lb, lt)), block(64);
+        unsigned long long* cnt = g.d_bcnt;
         switch (g.variant) {
         case 1:
-            L(&k_bounce_multi<WQ3D_PROFILE_MUNK, true>, grid, tpb, kps, g.d_ids, g.d_first, ng, S, b->clk);
+            L(&k_bounce_multi<WQ3D_PROFILE_MUNK, true>, grid, block, kps, g.d_ids, S, clk, blist, cnt, lb, lt);
             break;
         case 2:
-            L(&k_bounce_multi<WQ3D_PROFILE_LINEAR, true>, grid, tpb, kps, g.d_ids, g.d_first, ng, S, b->clk);
+            L(&k_bounce_multi<WQ3D_PROFILE_LINEAR, true>, grid, block, kps, g.d_ids, S, clk, blist, cnt, lb, lt);
             break;
         case 3:
-            L(&k_bounce_multi<kProfileGrid1Linear, true>, grid, tpb, kps, g.d_ids, g.d_first, ng, S, b->clk);
+            L(&k_bounce_multi<kProfileGrid1Linear, true>, grid, block, kps, g.d_ids, S, clk, blist, cnt, lb, lt);
             break;
         default:
             return fail(WQ3D_ERR_UNSUPPORTED, "environment kind without a batch variant");
@@ -202,14 +213,67 @@ wq3d_status batch_launch_step(wq3d_batch* b, Launcher& L, const Slots& S, int rs
     return WQ3D_OK;
 }
 #else
-// the emulation build steps the fans one after another through the
-// single-fan launch paths, with the same slots, times and record sets
+// The reflection launches of one group as the device runs them: every
+// thread of k_reflect_light_multi's grid, then every thread of
+// k_bounce_multi's, with the group's die lists in its device list array
+// (their slots checked against the grid) and the fans' host clocks.
+void emu_group_reflections(wq3d_batch* b, const wq3d_batch::Group& g, const Slots& S, int rset) {
+    const int64_t tpb = b->shape.ray_threads, blocks = g.blocks;
+    emu_range(g.d_blist, g.slots, "group reflection list");
+    unsigned long long cnt[2 * kXcds] = {};
+    auto cap_of = [&](int x) { return die_blocks(blocks, x) * tpb; };
+    auto slot = [&](int x, int64_t k) {
+        const int64_t at = die_slot(x, k, tpb);
+        if (k < 0 || k >= cap_of(x) || at < 0 || at >= blocks * tpb || at >= g.slots) {
+            std::fprintf(stderr, "wq3d emulation check failed: reflection list slot out of range\n");
+            std::abort();
+        }
+        return at;
+    };
+    for (int64_t w = 0; w < blocks; ++w) {
+        const int gi = (int)(std::upper_bound(g.first.begin(), g.first.end(), w) - g.first.begin()) - 1;
+        const KParams* P = b->fans[g.ids[gi]]->kp + rset;
+        emu::check_range(P, sizeof(KParams), "KParams");
+        const int x = (int)(w % kXcds);
+        for (int64_t t = 0; t < tpb; ++t) {
+            const int64_t i = (w - g.first[gi]) * tpb + t;
+            if (i == 0) {
+                emu_check_launch(P->E, P->F, P->A, &P->R);
+                *P->R.cand_count = 0ull;
+                *P->R.hit_count = 0ull;
+                *P->R.pass_count = 0ull;
+            }
+            if (i >= P->F.N) continue;
+            body_reflect_light(i, P->F, P->A, S);
+            if (!bounce_wanted(P->E, P->F, P->A, i)) continue;
+            const unsigned long long e = ((unsigned long long)gi << 32) | (unsigned long long)i;
+            if (bounce_side(P->A, i) == 0)
+                g.d_blist[slot(x, (int64_t)cnt[2 * x]++)] = e;
+            else
+                g.d_blist[slot(x, cap_of(x) - 1 - (int64_t)cnt[2 * x + 1]++)] = e;
+        }
+    }
+    const int64_t bthreads = (int64_t)bounce_grid(blocks, tpb) * 64;
+    for (int64_t gid = 0; gid < bthreads; ++gid) {
+        const int x = (int)((gid / 64) % kXcds);
+        const unsigned long long* c = cnt + 2 * x;
+        const int64_t j = (gid / 64 / kXcds) * 64 + gid % 64;
+        if (j >= (int64_t)(c[0] + c[1])) continue;
+        const unsigned long long e = g.d_blist[slot(x, die_entry(j, c, cap_of(x)))];
+        const int gi = (int)(e >> 32);
+        if (gi < 0 || gi >= (int)g.ids.size()) std::abort();
+        wq3d_fan* f = b->fans[g.ids[gi]];
+        const KParams* P = f->kp + rset;
+        body_bounce((int64_t)(e & 0xffffffffull), P->E, P->F, P->A, S, P->R, f->time, f->steps);
+    }
+}
+
+// the emulation build runs each group's reflections as above, then steps
+// the fans one after another through the single-fan launch paths, with the
+// same slots, times and record sets
 wq3d_status batch_launch_step(wq3d_batch* b, const Slots& S, int rset) {
     const Slots R{S.prev, S.curr, S.next, S.past};
-    for (wq3d_fan* f : b->fans) {
-        const unsigned grid = (unsigned)((f->F.N + kBlock - 1) / kBlock);
-        launch_bounce(b->stream, grid, f->kp + rset, f->env->dev, f->F, f->A, S, f->R, f->time, f->steps);
-    }
+    for (const auto& g : b->groups) emu_group_reflections(b, g, S, rset);
     for (wq3d_fan* f : b->fans) {
         const unsigned grid = (unsigned)((f->F.N + kBlock - 1) / kBlock);
         launch_advance(b->stream, grid, f->env->dev, f->F, f->A, R);
@@ -329,6 +393,10 @@ wq3d_status batch_prepare(wq3d_batch* b) {
         clk[k] = FanClock{f->time, f->steps, f->F.dt, 0};
     }
     HIP_TRY(rt::memcpy(b->clk, clk.data(), n * sizeof(FanClock), hipMemcpyHostToDevice));
+    // the reflection lists' counts: the clocks' parity restarts from the
+    // fans' step counts, which may have moved on without the batch
+    for (auto& g : b->groups)
+        if (g.d_bcnt) HIP_TRY(rt::memset_async(g.d_bcnt, 0, kListCounts * sizeof(unsigned long long), b->stream));
     return WQ3D_OK;
 }
 
@@ -424,9 +492,14 @@ wq3d_status wq3d_batch_create(wq3d_fan* const* fans, int32_t num_fans, wq3d_batc
         it->ids.push_back(k);
     }
     for (auto& g : b->groups) {
+        for (int32_t k : g.ids) g.rays += b->fans[k]->F.N;
+        g.slots = g.rays + 256 * (int64_t)g.ids.size();
         if (put(b->mem, &g.d_ids, g.ids.data(), g.ids.size()) != hipSuccess ||
             b->mem.alloc(&g.d_first, g.ids.size() + 1) != hipSuccess)
             return bail(fail(WQ3D_ERR_DEVICE, "batch table upload failed"));
+        if (b->mem.alloc(&g.d_blist, (size_t)g.slots) != hipSuccess ||
+            b->mem.alloc(&g.d_bcnt, kListCounts) != hipSuccess)
+            return bail(fail(WQ3D_ERR_DEVICE, "batch reflection list allocation failed"));
     }
     // the eigenray search: the fans with targets, each a whole grid of its own
     std::vector<int64_t> sup{0}, det{0};

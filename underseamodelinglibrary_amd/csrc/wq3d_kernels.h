@@ -585,6 +585,16 @@ WQ_HD bool try_bottom(RayCtx& x) {
     return true;
 }
 
+// the rays the reflection part of detect_reflections must visit: those that
+// reflect this step, or every ray when volume layers emit eigenverbs
+WQ_HD_INLINE bool bounce_wanted(const DEnv& E, const FanConst& F, const Arrays& A, int64_t i) {
+    return A.flag[i] != 0 || (F.want_verbs && E.nvol > 0);
+}
+
+// The reflection list a wanted ray goes to in a fused batch step: 1 for a
+// bottom reflection, 0 otherwise.
+WQ_HD_INLINE int bounce_side(const Arrays& A, int64_t i) { return A.flag[i] == 2 ? 1 : 0; }
+
 // detect_reflections (wave_queue.cc:197-292) in two parts.  The first runs
 // for every ray: the caustic test against the lower DE neighbour and, for
 // rays that do not reflect this step, the vertex counters.  The second
