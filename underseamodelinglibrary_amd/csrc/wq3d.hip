@@ -481,7 +481,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
 // One launch covers every fan of the batch: each fan owns a contiguous range
 // of the launch's blocks (first[g] .. first[g + 1] for the g-th fan of the
 // launch's list, fan id ids[g]), so a block belongs to one fan and runs that
-// fan's body with the fan's parameters (kps[fan], its device copy).  The
+// fan's body with the fan's parameters (kps[fan]: the batch's copies of the
+// fans' parameters in one array, so that the kernels read them through a
+// restrict kernel argument as scalar loads into scalar registers; read
+// through each fan's own copy, the fused per-ray kernels held them in
+// vector registers and reloaded them in their loops).  The
 // fans' times and step counts live in device memory (clk): k_bounce_multi
 // reads them before the step's rotation, k_advance_multi advances them with
 // the host's accumulation (time += dt), k_build_multi reads them after, as
@@ -513,14 +517,14 @@ __device__ __forceinline__ void wave_append(bool want, unsigned long long* count
 // The group's reflection lists (above) hold (fan of the group << 32 | ray);
 // the step parity is the fans' common step count before the rotation, read
 // from the device clocks so that a replayed graph reads the current one.
-__global__ void __launch_bounds__(kBlock) k_reflect_light_multi(const KParams* const* __restrict__ kps,
+__global__ void __launch_bounds__(kBlock) k_reflect_light_multi(const KParams* __restrict__ kps,
                                                                const int32_t* __restrict__ ids,
                                                                const int64_t* __restrict__ first, int n,
                                                                Slots S, const FanClock* __restrict__ clk,
                                                                unsigned long long* __restrict__ blist,
                                                                unsigned long long* __restrict__ bcnt) {
     const int g = fan_of_block(first, n, blockIdx.x);
-    const KParams* P = kps[ids[g]];
+    const KParams* P = kps + ids[g];
     const int64_t i = (int64_t)(blockIdx.x - first[g]) * blockDim.x + threadIdx.x;
     if (i == 0) {
         *P->R.cand_count = 0ull;
@@ -547,7 +551,7 @@ __global__ void __launch_bounds__(kBlock) k_reflect_light_multi(const KParams* c
 // leave after reading the lengths (a grid-stride loop instead doubles the
 // kernel's spills)
 template <int PK, bool FLAT>
-__global__ void __launch_bounds__(64) k_bounce_multi(const KParams* const* __restrict__ kps,
+__global__ void __launch_bounds__(64) k_bounce_multi(const KParams* __restrict__ kps,
                                                     const int32_t* __restrict__ ids, Slots S,
                                                     const FanClock* __restrict__ clk,
                                                     const unsigned long long* __restrict__ blist,
@@ -562,19 +566,19 @@ __global__ void __launch_bounds__(64) k_bounce_multi(const KParams* const* __res
     const int64_t k = die_entry(j, c, die_blocks(lblocks, x) * lthreads);
     const unsigned long long e = blist[die_slot(x, k, lthreads)];
     const int f = ids[(int)(e >> 32)];
-    const KParams* P = kps[f];
+    const KParams* P = kps + f;
     body_bounce<KindsOf<PK, FLAT>>((int64_t)(e & 0xffffffffull), P->E, P->F, P->A, S, P->R, clk[f].time,
                                    clk[f].step);
 }
 
 template <int PK, bool FLAT>
-__global__ void __launch_bounds__(kBlock) k_advance_multi(const KParams* const* __restrict__ kps,
+__global__ void __launch_bounds__(kBlock) k_advance_multi(const KParams* __restrict__ kps,
                                                          const int32_t* __restrict__ ids,
                                                          const int64_t* __restrict__ first, int n, Slots S,
                                                          FanClock* __restrict__ clk) {
     const int g = fan_of_block(first, n, blockIdx.x);
     const int f = ids[g];
-    const KParams* P = kps[f];
+    const KParams* P = kps + f;
     const int64_t i = (int64_t)(blockIdx.x - first[g]) * blockDim.x + threadIdx.x;
     if (i == 0) {
         clk[f].time += clk[f].dt;
@@ -586,24 +590,24 @@ __global__ void __launch_bounds__(kBlock) k_advance_multi(const KParams* const* 
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_supmask_multi(const KParams* const* __restrict__ kps,
+__global__ void __launch_bounds__(kBlock) k_supmask_multi(const KParams* __restrict__ kps,
                                                          const int32_t* __restrict__ ids,
                                                          const int64_t* __restrict__ first, int n) {
     const int g = fan_of_block(first, n, blockIdx.x);
-    const KParams* P = kps[ids[g]];
+    const KParams* P = kps + ids[g];
     supmask_block(P->F, P->A, P->R, win_tiles(P->F), blockIdx.x - first[g]);
 }
 
 // each fan's range is a whole detect grid, a multiple of kXcds blocks, so a
 // block's die (its index mod kXcds) is the one the single-fan grid gives it
-// Three wavefronts per SIMD, as the single-fan k_detect gets uncapped: the
-// fan lookup raises the fused kernel to 178 VGPRs (two wavefronts); the cap
-// spills 24 of them, and the search stage is 11 % faster.
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) k_detect_multi(const KParams* const* __restrict__ kps,
+// Three wavefronts per SIMD, as the single-fan k_detect: 160 VGPRs and no
+// spills with the parameters in scalar registers (read through the fan's
+// own copy they took 178 VGPRs, and the cap spilled 32 of them).
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) k_detect_multi(const KParams* __restrict__ kps,
                                                         const int32_t* __restrict__ ids,
                                                         const int64_t* __restrict__ first, int n, Slots S) {
     const int g = fan_of_block(first, n, blockIdx.x);
-    const KParams* P = kps[ids[g]];
+    const KParams* P = kps + ids[g];
     const int64_t tiles_az = (P->F.naz + kTile - 1) / kTile;
     const int64_t tiles = ((P->F.nde + kTile - 1) / kTile) * tiles_az;
     detect_block(P->F, P->A, S, P->R, (tiles + 3) / 4, blockIdx.x - first[g]);
@@ -613,13 +617,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
 // and each group of `group` lanes takes candidates of the whole list
 
 // two wavefronts per SIMD, as k_build
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) k_build_multi(const KParams* const* __restrict__ kps,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) k_build_multi(const KParams* __restrict__ kps,
                                                        const int32_t* __restrict__ ids, int n, Slots S,
                                                        const FanClock* __restrict__ clk, int group) {
     __shared__ int64_t s_first[kMaxBatchFans + 1];
     __shared__ double s_cube[(kBlock / kBuildLanes) * kBuildLds];
     for (int g = threadIdx.x; g < n; g += blockDim.x) {
-        const Records& R = kps[ids[g]]->R;
+        const Records& R = kps[ids[g]].R;
         const unsigned long long raw = *R.cand_count;
         s_first[g + 1] = (int64_t)(raw < (unsigned long long)R.cand_cap ? raw : (unsigned long long)R.cand_cap);
     }
@@ -636,7 +640,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
     for (int64_t c = gid / group; c < total; c += (int64_t)gridDim.x * blockDim.x / group) {
         const int g = fan_of_block(s_first, n, c);
         const int f = ids[g];
-        const KParams* P = kps[f];
+        const KParams* P = kps + f;
         body_build<true>(c - s_first[g], lane_in_group, group, lds, P->E, P->F, P->A, S, P->R, clk[f].time,
                          clk[f].step);
     }

@@ -81,7 +81,7 @@ struct wq3d_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     DeviceBuffers mem;
-    const KParams** kps = nullptr;  // device [2][n]: fan f's parameters for record set r at r * n + f
+    KParams* kps = nullptr;         // device [2][n]: copy of fan f's parameters for record set r at r * n + f
     FanClock* clk = nullptr;        // device [n]
     struct Group {                  // fans of one environment variant (per-ray launches)
         int variant = 0;
@@ -142,7 +142,7 @@ wq3d_status batch_tables(wq3d_batch* b) {
 #ifndef WQ3D_EMULATE
 wq3d_status batch_launch_step(wq3d_batch* b, Launcher& L, const Slots& S, int rset) {
     const int n = (int)b->fans.size();
-    const KParams* const* kps = b->kps + (size_t)rset * n;
+    const KParams* kps = b->kps + (size_t)rset * n;
     const dim3 tpb((unsigned)b->shape.ray_threads);
     const FanClock* clk = b->clk;
     for (const auto& g : b->groups)
@@ -370,13 +370,15 @@ wq3d_status batch_check_aligned(const wq3d_batch* b) {
     return WQ3D_OK;
 }
 
-// the fans' parameter copies and clocks as the host holds them, before a
-// batch call's first launch (the stream is idle between calls)
+// the fans' parameter copies (each fan's own and the batch's) and clocks as
+// the host holds them, before a batch call's first launch (the stream is
+// idle between calls)
 wq3d_status batch_prepare(wq3d_batch* b) {
     wq3d_status st = batch_check_aligned(b);
     if (st) return st;
     const size_t n = b->fans.size();
     std::vector<FanClock> clk(n);
+    std::vector<KParams> hk(2 * n);
     for (size_t k = 0; k < n; ++k) {
         wq3d_fan* f = b->fans[k];
         if (f->kp_dirty) {
@@ -391,7 +393,14 @@ wq3d_status batch_prepare(wq3d_batch* b) {
             f->kp_dirty = false;
         }
         clk[k] = FanClock{f->time, f->steps, f->F.dt, 0};
+        const int active = f->rset;
+        for (int r = 0; r < 2; ++r) {
+            use_record_set(f, r);
+            hk[(size_t)r * n + k] = KParams{f->env->dev, f->F, f->A, f->R};
+        }
+        use_record_set(f, active);
     }
+    HIP_TRY(rt::memcpy(b->kps, hk.data(), hk.size() * sizeof(KParams), hipMemcpyHostToDevice));
     HIP_TRY(rt::memcpy(b->clk, clk.data(), n * sizeof(FanClock), hipMemcpyHostToDevice));
     // the reflection lists' counts: the clocks' parity restarts from the
     // fans' step counts, which may have moved on without the batch
@@ -469,12 +478,7 @@ wq3d_status wq3d_batch_create(wq3d_fan* const* fans, int32_t num_fans, wq3d_batc
     if (rt::set_device(b->device) != hipSuccess || rt::stream_create(&b->stream) != hipSuccess)
         return bail(fail(WQ3D_ERR_DEVICE, "batch stream creation failed"));
     const int n = num_fans;
-    std::vector<const KParams*> kps((size_t)2 * n);
-    for (int k = 0; k < n; ++k) {
-        kps[k] = b->fans[k]->kp + 0;
-        kps[(size_t)n + k] = b->fans[k]->kp + 1;
-    }
-    if (put(b->mem, &b->kps, kps.data(), kps.size()) != hipSuccess || b->mem.alloc(&b->clk, (size_t)n) != hipSuccess)
+    if (b->mem.alloc(&b->kps, (size_t)2 * n) != hipSuccess || b->mem.alloc(&b->clk, (size_t)n) != hipSuccess)
         return bail(fail(WQ3D_ERR_DEVICE, "batch table upload failed"));
     // per-ray launches: fans grouped by environment variant
     for (int k = 0; k < n; ++k) {
