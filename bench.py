@@ -45,7 +45,7 @@ W_BOX = 59.0              # FP64 ops per (ray, box) bound: the window form bound
 # k_detect_multi) on this workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE
 # x 1.0 (the factors a 1 GiB device copy gives), means over the 4874 launches
 # of a run; profiles/search_stage_config5_32_profiles.md holds the commands
-SEARCH_TRAFFIC_BYTES = ((65107.0 + 86484.0) * 2.0 + (1249.0 + 168714.0)) * 1024.0
+SEARCH_TRAFFIC_BYTES = ((65107.0 + 75768.0) * 2.0 + (1249.0 + 2334.0)) * 1024.0
 
 
 def log(msg):
