@@ -334,57 +334,109 @@ __device__ __forceinline__ void detect_block(const FanConst& F, const Arrays& A,
         qn = 0;
         wave_sync();
     };
-    // one target's pair test for every lane's ray, the later forms only
-    // while some lane is still in; the pairs the filter keeps go to the
-    // candidate list, whose exact tests k_build makes
-    auto visit = [&](bool in, const double* f, int64_t t, bool bbr) {
-        if (!__any(in)) return;
-        in = in && detect_window_prev_ok(C.W, f);
-        if (!__any(in)) return;
-        const bool tbr = bbr && F.scan_branch[t] != 0;
-        in = in && (tbr || detect_cell_nb_ok(C, f));
+    // The pair tests run with one target of the block per lane and the
+    // wavefront's kept rays in turn (popc(keep) window forms instead of one
+    // target per round for all 64 ray lanes, about 40 % of which keep a
+    // block): each ray's window form is read from LDS as a broadcast.  The
+    // pairs inside a window (about 1.5 % of those tested) are queued and
+    // finished with one pair per lane, the ray's later forms fetched from
+    // its lane: the previous-time form, then, while some lane is still in,
+    // the neighbour forms.  The pairs the filter keeps go to the candidate
+    // list, whose exact tests k_build makes.
+    __shared__ double s_w[kBlock / 64][64][8];  // the rays' window forms {k, a[6], w}
+    __shared__ uint32_t s_q[kBlock / 64][64];   // queued pairs: ray lane << 8 | target offset
+    s_w[wv][lane][0] = C.W.form.k;
+    for (int j = 0; j < 6; ++j) s_w[wv][lane][1 + j] = C.W.form.a[j];
+    s_w[wv][lane][7] = C.W.w;
+    const double m = F.lin_margin;  // every searched ray's rounding margin
+    unsigned nq = 0;                // queued pairs
+    // the queued pairs of the block at t0 with target branch flags bbr
+    auto finish = [&](int64_t t0, bool bbr) {
+        wave_sync();  // the queue's writes come first
+        const bool has = (unsigned)lane < nq;
+        const uint32_t e = has ? s_q[wv][lane] : 0u;
+        const int src = has ? (int)(e >> 8) : lane;
+        const int64_t t = t0 + (int64_t)(e & 0xffu);
+        double f[6];
+        for (int j = 0; j < 6; ++j) f[j] = has ? F.tgt_lin[t * 6 + j] : 0.0;
+        LinForm pv;
+        pv.k = __shfl(C.W.prev.k, src);
+        for (int j = 0; j < 6; ++j) pv.a[j] = __shfl(C.W.prev.a[j], src);
+        bool in = has && !(lin_eval(pv, f) < -m);
+        const bool tbr = in && bbr && F.scan_branch[t] != 0;
+        bool nb_in = in && !tbr;
+        if (__any(nb_in)) {
+            const uint32_t use = (uint32_t)__shfl((int)C.use, src);
+            for (int n = 0; n < 4; ++n) {
+                LinForm nf;
+                nf.k = __shfl(C.nb[n].k, src);
+                for (int j = 0; j < 6; ++j) nf.a[j] = __shfl(C.nb[n].a[j], src);
+                if (nb_in && ((use >> n) & 1u) && lin_eval(nf, f) < -m) in = false;
+            }
+        }
+        const long long rsrc = __shfl((long long)ray, src);
         const unsigned long long hm = __ballot(in);
-        if (!hm) return;
-        const unsigned cnt = (unsigned)__popcll(hm);
-        hits += cnt;
-        if (qn + cnt > (unsigned)kHitQueue) flush();
-        if (in)
-            s_hit[wv][qn + (unsigned)__popcll(hm & ((1ull << lane) - 1ull))] =
-                ((unsigned long long)F.tgt_perm[t] << 32) | (unsigned long long)ray;
-        qn += cnt;
+        if (hm) {
+            const unsigned cnt = (unsigned)__popcll(hm);
+            hits += cnt;
+            if (qn + cnt > (unsigned)kHitQueue) flush();
+            if (in)
+                s_hit[wv][qn + (unsigned)__popcll(hm & ((1ull << lane) - 1ull))] =
+                    ((unsigned long long)F.tgt_perm[t] << 32) | (unsigned long long)rsrc;
+            qn += cnt;
+        }
+        nq = 0;
+        wave_sync();  // the queue's reads come before its next writes
     };
-    // this wavefront's copy of the block of targets being tested: one
-    // vector load per lane instead of a chain of scalar loads per target
-    __shared__ LinFeatures s_tf[kBlock / 64][kTargetBlock];
-    const LinFeatures* lf = s_tf[wv];
+    // queues the pairs (ray lane r, this lane's target) whose window test
+    // passed
+    auto enqueue = [&](bool pass, int r, int64_t t0, bool bbr) {
+        const unsigned long long pm = __ballot(pass);
+        if (!pm) return;
+        const unsigned cnt = (unsigned)__popcll(pm);
+        if (nq + cnt > 64u) finish(t0, bbr);
+        if (pass) s_q[wv][nq + (unsigned)__popcll(pm & ((1ull << lane) - 1ull))] = ((uint32_t)r << 8) | (uint32_t)lane;
+        nq += cnt;
+    };
+    wave_sync();  // the window forms' writes come before their reads
     for (int q = 0; q < nbk; ++q) {
         const unsigned long long kq = keep[q];
         if (!kq) continue;
-        const bool need = (kq >> lane) & 1ull;
         const int64_t t0 = (b_begin + q) * kTargetBlock;
         const int64_t t_end = (F.nt < t0 + kTargetBlock) ? F.nt : t0 + kTargetBlock;
         tested += (unsigned long long)__popcll(kq) * (unsigned long long)(t_end - t0);
         const bool bbr = s_br[q] != 0;
-        wave_sync();  // the previous block's reads come first
-        if (t0 + lane < t_end) s_tf[wv][lane] = reinterpret_cast<const LinFeatures*>(F.tgt_lin)[t0 + lane];
-        wave_sync();
-        // three targets per round (the same for every lane, read from LDS),
-        // three window forms as independent chains
-        int64_t t = t0;
-        for (; t + 3 <= t_end; t += 3) {
-            const LinFeatures f0 = lf[t - t0], f1 = lf[t + 1 - t0], f2 = lf[t + 2 - t0];
-            const bool i0 = need && detect_window_in(C.W, f0.f);
-            const bool i1 = need && detect_window_in(C.W, f1.f);
-            const bool i2 = need && detect_window_in(C.W, f2.f);
-            if (!__any(i0 || i1 || i2)) continue;
-            visit(i0, f0.f, t, bbr);
-            visit(i1, f1.f, t + 1, bbr);
-            visit(i2, f2.f, t + 2, bbr);
+        // this lane's target
+        const bool tv = t0 + lane < t_end;
+        double f[6];
+        for (int j = 0; j < 6; ++j) f[j] = tv ? F.tgt_lin[(t0 + lane) * 6 + j] : 0.0;
+        auto window_in = [&](int r) {
+            const double* w = s_w[wv][r];
+            double d = ::fma(w[1], f[0], w[0]);
+            for (int j = 1; j < 6; ++j) d = ::fma(w[1 + j], f[j], d);
+            return !(fabs(d) > w[7]);
+        };
+        // the kept rays, three per round as independent chains with no
+        // branch between them (a round short of rays repeats its first,
+        // whose result is masked)
+        unsigned long long rem = kq;
+        while (rem) {
+            const int r0 = __ffsll((long long)rem) - 1;
+            rem &= rem - 1ull;
+            const bool h1 = rem != 0ull;
+            const int r1 = h1 ? __ffsll((long long)rem) - 1 : r0;
+            rem &= h1 ? rem - 1ull : ~0ull;
+            const bool h2 = rem != 0ull;
+            const int r2 = h2 ? __ffsll((long long)rem) - 1 : r0;
+            rem &= h2 ? rem - 1ull : ~0ull;
+            const bool w0 = window_in(r0), w1 = window_in(r1), w2 = window_in(r2);
+            const bool p0 = tv & w0, p1 = tv & h1 & w1, p2 = tv & h2 & w2;
+            if (!__any(p0 | p1 | p2)) continue;
+            enqueue(p0, r0, t0, bbr);
+            enqueue(p1, r1, t0, bbr);
+            enqueue(p2, r2, t0, bbr);
         }
-        for (; t < t_end; ++t) {
-            const LinFeatures fc = lf[t - t0];
-            visit(need && detect_window_in(C.W, fc.f), fc.f, t, bbr);
-        }
+        if (nq) finish(t0, bbr);
     }
     if (qn) flush();
     if (lane == 0) {
