@@ -62,6 +62,9 @@ static_assert(kMaxFreq <= kBuildLanes && kBlock % kBuildLanes == 0 && 64 % kBuil
 // flush
 constexpr int kTile = kWinTile;
 constexpr int kHitQueue = 128;
+// k_detect: a block kept by more rays of a wavefront than this is tested in
+// ray lanes, target by target; fewer, in target lanes, ray by ray
+constexpr int kDenseKeep = 40;
 // MI355X accelerator dies (XCDs), each with its own L2; consecutive
 // workgroups of a launch are dispatched to them in turn
 constexpr int kXcds = 8;
@@ -334,10 +337,12 @@ __device__ __forceinline__ void detect_block(const FanConst& F, const Arrays& A,
         qn = 0;
         wave_sync();
     };
-    // The pair tests run with one target of the block per lane and the
-    // wavefront's kept rays in turn (popc(keep) window forms instead of one
-    // target per round for all 64 ray lanes, about 40 % of which keep a
-    // block): each ray's window form is read from LDS as a broadcast.  The
+    // A block most of whose rays keep it (more than kDenseKeep) is tested
+    // one target per round against the ray lanes.  The others run with one
+    // target of the block per lane and the wavefront's kept rays in turn
+    // (popc(keep) window forms instead of one target per round for all 64
+    // ray lanes; on config 5 about 40 % of the lanes keep a block): each
+    // ray's window form is read from LDS as a broadcast.  The
     // pairs inside a window (about 1.5 % of those tested) are queued and
     // finished with one pair per lane, the ray's later forms fetched from
     // its lane: the previous-time form, then, while some lane is still in,
@@ -398,6 +403,27 @@ __device__ __forceinline__ void detect_block(const FanConst& F, const Arrays& A,
         if (pass) s_q[wv][nq + (unsigned)__popcll(pm & ((1ull << lane) - 1ull))] = ((uint32_t)r << 8) | (uint32_t)lane;
         nq += cnt;
     };
+    // the dense blocks' pair test: one target for every lane's ray, the
+    // later forms only while some lane is still in
+    auto visit = [&](bool in, const double* f, int64_t t, bool bbr) {
+        if (!__any(in)) return;
+        in = in && detect_window_prev_ok(C.W, f);
+        if (!__any(in)) return;
+        const bool tbr = bbr && F.scan_branch[t] != 0;
+        in = in && (tbr || detect_cell_nb_ok(C, f));
+        const unsigned long long hm = __ballot(in);
+        if (!hm) return;
+        const unsigned cnt = (unsigned)__popcll(hm);
+        hits += cnt;
+        if (qn + cnt > (unsigned)kHitQueue) flush();
+        if (in)
+            s_hit[wv][qn + (unsigned)__popcll(hm & ((1ull << lane) - 1ull))] =
+                ((unsigned long long)F.tgt_perm[t] << 32) | (unsigned long long)ray;
+        qn += cnt;
+    };
+    // a dense block's targets, for every lane to read (LDS broadcast)
+    __shared__ LinFeatures s_tf[kBlock / 64][kTargetBlock];
+    const LinFeatures* lf = s_tf[wv];
     wave_sync();  // the window forms' writes come before their reads
     for (int q = 0; q < nbk; ++q) {
         const unsigned long long kq = keep[q];
@@ -406,6 +432,30 @@ __device__ __forceinline__ void detect_block(const FanConst& F, const Arrays& A,
         const int64_t t_end = (F.nt < t0 + kTargetBlock) ? F.nt : t0 + kTargetBlock;
         tested += (unsigned long long)__popcll(kq) * (unsigned long long)(t_end - t0);
         const bool bbr = s_br[q] != 0;
+        if (__popcll(kq) > kDenseKeep) {
+            // most rays keep the block: targets in turn, three per round,
+            // for the ray lanes that keep it
+            const bool need = (kq >> lane) & 1ull;
+            wave_sync();  // the previous block's reads come first
+            if (t0 + lane < t_end) s_tf[wv][lane] = reinterpret_cast<const LinFeatures*>(F.tgt_lin)[t0 + lane];
+            wave_sync();
+            int64_t t = t0;
+            for (; t + 3 <= t_end; t += 3) {
+                const LinFeatures f0 = lf[t - t0], f1 = lf[t + 1 - t0], f2 = lf[t + 2 - t0];
+                const bool i0 = need && detect_window_in(C.W, f0.f);
+                const bool i1 = need && detect_window_in(C.W, f1.f);
+                const bool i2 = need && detect_window_in(C.W, f2.f);
+                if (!__any(i0 || i1 || i2)) continue;
+                visit(i0, f0.f, t, bbr);
+                visit(i1, f1.f, t + 1, bbr);
+                visit(i2, f2.f, t + 2, bbr);
+            }
+            for (; t < t_end; ++t) {
+                const LinFeatures fc = lf[t - t0];
+                visit(need && detect_window_in(C.W, fc.f), fc.f, t, bbr);
+            }
+            continue;
+        }
         // this lane's target
         const bool tv = t0 + lane < t_end;
         double f[6];
@@ -445,7 +495,8 @@ __device__ __forceinline__ void detect_block(const FanConst& F, const Arrays& A,
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_detect(FanConst F, Arrays A, Slots S, Records Rc, int64_t quads) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) k_detect(FanConst F, Arrays A, Slots S,
+                                                                                         Records Rc, int64_t quads) {
     detect_block(F, A, S, Rc, quads, blockIdx.x);
 }
 
