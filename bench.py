@@ -350,7 +350,7 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--legs", default="config1,config2fan,collection,sources,adjoint")
-    ap.add_argument("--adjoint-workers", type=int, default=4)
+    ap.add_argument("--adjoint-workers", type=int, default=8)
     ap.add_argument("--adjoint-profiles", type=int, default=None,
                     help="profiles in the adjoint leg (default: --profiles)")
     ap.add_argument("--sources", type=int, default=256, help="config 4 sources in the batch leg")

@@ -64,7 +64,7 @@ constexpr int kTile = kWinTile;
 constexpr int kHitQueue = 128;
 // k_detect: a block kept by more rays of a wavefront than this is tested in
 // ray lanes, target by target; fewer, in target lanes, ray by ray
-constexpr int kDenseKeep = 40;
+constexpr int kDenseKeep = 32;
 // MI355X accelerator dies (XCDs), each with its own L2; consecutive
 // workgroups of a launch are dispatched to them in turn
 constexpr int kXcds = 8;
