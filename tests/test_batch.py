@@ -191,3 +191,53 @@ def test_batch_equals_each_alone_gpu(mode):
     counts = run_batch_vs_alone(lib, mode)
     print(f"[batch] eigenrays per fan {counts}", flush=True)
     assert counts[0] > 0 and counts[2] > 0 and counts[3] >= 3
+
+
+@pytest.mark.gpu
+def test_batch_sees_parameter_changes_between_calls_gpu():
+    """The fused kernels read the batch's own copies of the fans' parameters;
+    a fan's thresholds and listener flags changed between two batch calls
+    must reach them, as wq3d_step sees them on the fan stepped alone."""
+    import ctypes as C
+    from underseamodelinglibrary_amd import _lib
+    from underseamodelinglibrary_amd.usml.waveq3d import fan_batch
+    lib = _lib.load()
+    cases = _cases()
+    first = 70
+    print(f"[batch] parameter changes after step {first}: {len(cases)} fans, {STEPS} steps", flush=True)
+
+    def change(qs):
+        th = _abi.Thresholds()
+        C.memmove(C.byref(th), C.byref(qs[0]._thr), C.sizeof(th))
+        th.max_bottom = 1
+        th.intensity = 150.0
+        lib.check(lib.wq3d_set_thresholds(qs[0]._fan, C.byref(th)))
+        lib.check(lib.wq3d_set_listeners(qs[1]._fan, 1, 1))
+        lib.check(lib.wq3d_set_thresholds(qs[2]._fan, C.byref(th)))
+
+    alone = [_queue(make(), lib, verbs) for _, make, verbs in cases]
+    for q in alone:
+        lib.check(lib.wq3d_step(q._fan, first))
+    change(alone)
+    for q in alone:
+        lib.check(lib.wq3d_step(q._fan, STEPS - first))
+    ref = [_records(q) for q in alone]
+    for q in alone:
+        q.close()
+    # the changes show: the second fan, created without listeners, now has
+    # eigenverbs
+    assert len(ref[1][3]) > 0
+    queues = [_queue(make(), lib, verbs) for _, make, verbs in cases]
+    batch = fan_batch(queues, lib=lib)
+    lib.check(lib.wq3d_batch_step(batch._h, first))
+    change(queues)
+    lib.check(lib.wq3d_batch_step(batch._h, STEPS - first))
+    for (name, _, _), q, r in zip(cases, queues, ref):
+        got = _records(q)
+        assert len(got) == len(r)
+        for k, (x, y) in enumerate(zip(got, r)):
+            assert x.dtype == y.dtype and x.shape == y.shape, (name, k)
+            assert x.tobytes() == y.tobytes(), (name, k)
+    batch.close()
+    for q in queues:
+        q.close()
