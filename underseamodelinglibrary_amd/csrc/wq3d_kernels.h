@@ -1177,8 +1177,14 @@ WQ_HD_INLINE bool load_cell_window(const FanConst& F, const Arrays& A, int64_t i
 WQ_HD_INLINE void load_cell_rest(const FanConst& F, const Arrays& A, const Slots& S, int64_t i, DetectCell& C) {
     C.use = 0u;
     if (!detect_interior(F, A, S, i)) {
+        // defined forms for every lane: k_detect reads another lane's forms
+        // with a shuffle that every lane takes part in
         C.W.prev.k = 0.0;
         for (int j = 0; j < 6; ++j) C.W.prev.a[j] = 0.0;
+        for (int n = 0; n < 4; ++n) {
+            C.nb[n].k = 0.0;
+            for (int j = 0; j < 6; ++j) C.nb[n].a[j] = 0.0;
+        }
         return;
     }
     const int64_t N = F.N;
