@@ -204,17 +204,22 @@ def leg_config2_fan(lib):
     lib.check(lib.wq3d_step(q._fan, 20))
     q.sync()
     q.drain_eigenrays()
+    pairs, boxes = C.c_int64(), C.c_int64()
+    lib.check(lib.wq3d_search_totals(q._fan, C.byref(pairs), C.byref(boxes)))  # from here on
     steps = sc.steps - 20
     t0 = time.perf_counter()
     lib.check(lib.wq3d_step(q._fan, steps))
     q.sync()
     dt = time.perf_counter() - t0
+    lib.check(lib.wq3d_search_totals(q._fan, C.byref(pairs), C.byref(boxes)))
     recs, inten, phase = q.drain_eigenrays()
     q.close()
     return ({"workload": "config 2's fan in config 1's Munk ocean (off-config: the gridded ocean is not run on "
                          "the GPU): 641 DE x 180 AZ, 8 freqs 100-10000 Hz, 100x100 targets, steps 21-1201",
              "ray_steps_per_s": sc.num_rays() * steps / dt, "eigenrays": int(len(recs)),
-             "eigenrays_per_s": len(recs) / dt, "ms_per_step": dt * 1e3 / steps}, sc, recs, inten, phase)
+             "eigenrays_per_s": len(recs) / dt, "ms_per_step": dt * 1e3 / steps,
+             "pairs_tested_per_step": pairs.value / steps, "box_tests_per_step": boxes.value / steps},
+            sc, recs, inten, phase)
 
 
 def leg_collection(lib, sc, recs, inten, phase, cpu):
